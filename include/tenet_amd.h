@@ -1,0 +1,123 @@
+/*
+ * tenet_amd.h -- C ABI of the MI355X (gfx950) block-sparse contraction engine.
+ *
+ * Every entry point returns 0 on success and a nonzero code on failure; the message is
+ * read with tn_last_error().  No exception crosses the ABI.  All tensor and table
+ * pointers are DEVICE pointers owned by the caller (the Python host keeps them in torch
+ * tensors).  Every call takes a hipStream_t (passed as void*), enqueues asynchronously
+ * and never synchronizes; calls on distinct streams are independent.
+ *
+ * Data model: a block-sparse tensor is ONE flat buffer holding its blocks back to back in
+ * the reference's canonical block order, each block C-contiguous in public axis order.
+ * Plans (block offsets, permutations, recoupling coefficients, GEMM problem lists) are
+ * compiled once on the host and uploaded as int64/double tables.
+ *
+ * Reference interfaces replaced (paths in Ryo-wtnb11/TeNeT-py):
+ *   tn_recouple      <- the block transposes + coeff-scaled accumulation executed by
+ *                       ops/repartition.py:458-468, ops/permutation.py:254-264 and the
+ *                       gather/scatter of map_view.py:274-278 (to_matrices) and
+ *                       map_view.py:331-334 (from_matrices).
+ *   tn_gemm_grouped  <- the per-coupled-sector ar.do("matmul") loop of ops/map.py:160-161
+ *                       (compose), i.e. the GEMM batch under tenet.einsum / tensordot.
+ *   tn_dot, tn_axpby, tn_scale
+ *                    <- tenet.inner / tenet.norm / add / subtract / scalar multiply
+ *                       (ops/contraction.py:584-589, ops/basic.py:119,157,361-367) that
+ *                       network/dmrg.py:106-227 (lanczos) runs on.
+ *   tn_cast          <- ar.do("astype") on blocks (precision staging for FP32 GEMMs).
+ *   tn_mpo_apply     <- the two W contractions of network/env.py:430-431 (_heff2_full),
+ *                       fused into one pass with a host-merged two-site MPO table.
+ */
+#ifndef TENET_AMD_H
+#define TENET_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element types */
+#define TN_F64 0
+#define TN_F32 1
+
+/* GEMM arithmetic: FP64 MFMA, FP32 MFMA, and FP32 emulated by three BF16 MFMA products
+ * (hi*hi + hi*lo + lo*hi) with FP32 accumulation. */
+#define TN_PREC_FP64 0
+#define TN_PREC_FP32 1
+#define TN_PREC_BF16X3 2
+
+/* Recouple job table: TN_JOB_WORDS int64 per destination region.
+ *   [0] dst_off   [1] term_begin  [2] term_end  [3] rank (<= TN_MAX_RANK)
+ *   [4..9] shape  [10..15] dst_stride  [16..21] src_stride (dims in dst C order)
+ *   [22] mode: -1 = linear (dst-ordered), else the index of the src-contiguous dim used
+ *        as the second axis of a 64x64 LDS transpose tile
+ *   [23] element count
+ * Term table: per term one int64 source offset (elements) and one double coefficient.
+ * Tile table: 2 int64 per tile: (job, tile index within the job). */
+#define TN_JOB_WORDS 24
+#define TN_MAX_RANK 6
+
+/* GEMM problem table: TN_PROB_WORDS int64 per problem (one output block):
+ *   [0] M [1] N [2] c_off [3] ldc [4] seg_begin [5] seg_end [6] c_rows_stride_unused [7] 0
+ * Segment table: TN_SEG_WORDS int64 per K segment:
+ *   [0] a_off [1] b_off [2] K [3] lda [4] ldb [5] 0
+ * Tile table: 4 int32 per output tile: (problem, tile row, tile col, 0). */
+#define TN_PROB_WORDS 8
+#define TN_SEG_WORDS 6
+
+const char* tn_last_error(void);
+int tn_version(void);
+
+/* Number of compute units of the current device (for persistent-grid sizing). */
+int tn_device_cus(int* out);
+
+/* dst[job region] = sum_t coeff_t * permute(src + term_src_off_t), over every job. */
+int tn_recouple(int dtype, const void* src, void* dst, const int64_t* jobs,
+                const int64_t* term_src, const double* term_coeff, const int64_t* tiles,
+                int64_t ntiles, void* stream);
+
+/* For every problem p: C_p = sum_s op(A_s) op(B_s) with op = transpose when trans_a/b.
+ * A/B/C are base pointers; every offset in the tables is in elements.  bm x bn is the
+ * workgroup tile (one of 32x32, 64x64, 128x64, 64x128, 128x128); grid 0 = one workgroup
+ * per tile, otherwise a persistent loop over tiles. */
+int tn_gemm_grouped(int precision, int dtype, int trans_a, int trans_b, const void* A,
+                    const void* B, void* C, const int64_t* problems, const int64_t* segs,
+                    const int32_t* tiles, int64_t ntiles, int bm, int bn, int grid,
+                    void* stream);
+
+/* out[0] = sum_i w_i x_i y_i (w may be NULL for unit weights), FP64 accumulation,
+ * deterministic two-pass reduction; `work` must hold >= 1024 doubles. */
+int tn_dot(int dtype, const void* x, const void* y, const double* w, int64_t n,
+           double* work, double* out, void* stream);
+
+/* y = alpha * x + beta * y with alpha/beta read from device scalars when the pointer
+ * form is used (alpha_dev/beta_dev non-NULL) -- no host synchronization. The scalar
+ * used is  s_host * (s_dev ? *s_dev : 1) , optionally inverted when inv_* is set. */
+int tn_axpby(int dtype, int64_t n, double alpha, const double* alpha_dev, int inv_alpha,
+             const void* x, double beta, const double* beta_dev, int inv_beta, void* y,
+             void* stream);
+
+/* x *= s (s host scalar times optional device scalar, optionally inverted). */
+int tn_scale(int dtype, int64_t n, double s, const double* s_dev, int inv, void* x,
+             void* stream);
+
+/* dst = (dst_dtype) src elementwise. */
+int tn_cast(int src_dtype, const void* src, int dst_dtype, void* dst, int64_t n, void* stream);
+
+/* Fused two-site MPO application (the middle of the two-site H_eff).  For each job
+ * (a pair of outer slabs) with R rows and S contiguous columns:
+ *   out[o][r][s] = sum_k coef[o][k] * in[k][r][s]
+ * where in/out slabs are addressed by per-job offset lists.  Job table per job (int64):
+ *   [0] R [1] S [2] in_begin [3] in_end [4] out_begin [5] out_end [6] coef_off [7] 0
+ * in/out lists hold (offset, row_stride) int64 pairs; coef is a dense row-major
+ * (nout x nin) double table per job. */
+#define TN_MPO_JOB_WORDS 8
+int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t* jobs, int64_t njobs,
+                 const int64_t* in_slabs, const int64_t* out_slabs, const double* coef,
+                 const int64_t* tiles, int64_t ntiles, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENET_AMD_H */

@@ -1,0 +1,233 @@
+// Vector-space kernels for the Krylov solver, precision staging, the fused two-site MPO
+// application, and the C-ABI error state.
+//
+// tn_dot is a deterministic two-pass reduction in FP64; scalars that the Lanczos
+// recurrence needs stay in device memory so a matvec-orthogonalise step needs no host
+// round trip.
+#include <string>
+
+#include "tn_common.hpp"
+
+namespace tn {
+namespace {
+thread_local std::string g_error;
+}
+void set_error(const std::string& msg) { g_error = msg; }
+}  // namespace tn
+
+extern "C" const char* tn_last_error(void) { return tn::g_error.c_str(); }
+extern "C" int tn_version(void) { return 1; }
+
+extern "C" int tn_device_cus(int* out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return tn::fail("tn_device_cus: no HIP device");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return tn::fail("tn_device_cus: hipGetDeviceProperties failed");
+  *out = prop.multiProcessorCount;
+  return 0;
+}
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxPartials = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void dot_partial(const T* __restrict__ x,
+                                                        const T* __restrict__ y,
+                                                        const double* __restrict__ w, int64_t n,
+                                                        double* __restrict__ part) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
+    double v = static_cast<double>(x[i]) * static_cast<double>(y[i]);
+    if (w) v *= w[i];
+    acc += v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kThreads) void dot_final(const double* __restrict__ part, int np,
+                                                      double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += kThreads) acc += part[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__device__ __forceinline__ double scalar(double h, const double* d, int inv) {
+  if (!d) return h;
+  const double v = *d;
+  return h * (inv ? 1.0 / v : v);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void axpby_kernel(int64_t n, double a, const double* ad,
+                                                         int ainv, const T* __restrict__ x,
+                                                         double b, const double* bd, int binv,
+                                                         T* __restrict__ y) {
+  const double alpha = scalar(a, ad, ainv);
+  const double beta = scalar(b, bd, binv);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
+    const double yv = beta == 0.0 ? 0.0 : beta * static_cast<double>(y[i]);
+    y[i] = static_cast<T>(alpha * static_cast<double>(x[i]) + yv);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void scale_kernel(int64_t n, double s, const double* sd,
+                                                         int inv, T* __restrict__ x) {
+  const double f = scalar(s, sd, inv);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride)
+    x[i] = static_cast<T>(f * static_cast<double>(x[i]));
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(kThreads) void cast_kernel(const S* __restrict__ src, D* __restrict__ dst,
+                                                        int64_t n) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride)
+    dst[i] = static_cast<D>(src[i]);
+}
+
+unsigned grid_for(int64_t n) {
+  int64_t g = (n + kThreads * 4 - 1) / (kThreads * 4);
+  if (g < 1) g = 1;
+  if (g > 8192) g = 8192;
+  return static_cast<unsigned>(g);
+}
+
+// ---- fused two-site MPO application --------------------------------------------------
+constexpr int kMpoMax = 32;  // max slabs on either side of one job
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void mpo_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                       const int64_t* __restrict__ jobs,
+                                                       const int64_t* __restrict__ in_slabs,
+                                                       const int64_t* __restrict__ out_slabs,
+                                                       const double* __restrict__ coef,
+                                                       const int64_t* __restrict__ tiles) {
+  const int64_t job = tiles[2 * blockIdx.x];
+  const int64_t e = tiles[2 * blockIdx.x + 1] * kThreads + threadIdx.x;
+  const int64_t* jw = jobs + job * TN_MPO_JOB_WORDS;
+  const int64_t R = jw[0], S = jw[1];
+  if (e >= R * S) return;
+  const int64_t r = e / S, s = e - (e / S) * S;
+  const int nin = static_cast<int>(jw[3] - jw[2]);
+  const int nout = static_cast<int>(jw[5] - jw[4]);
+  if (nin > kMpoMax) return;  // the host refuses such tables; never index past v[]
+  const int64_t* is = in_slabs + 2 * jw[2];
+  const int64_t* os = out_slabs + 2 * jw[4];
+  const double* cf = coef + jw[6];
+  double v[kMpoMax];
+#pragma unroll 4
+  for (int k = 0; k < nin; ++k) v[k] = static_cast<double>(in[is[2 * k] + r * is[2 * k + 1] + s]);
+  for (int o = 0; o < nout; ++o) {
+    double acc = 0.0;
+    const double* row = cf + o * nin;
+    for (int k = 0; k < nin; ++k) acc += row[k] * v[k];
+    out[os[2 * o] + r * os[2 * o + 1] + s] = static_cast<T>(acc);
+  }
+}
+
+}  // namespace
+
+extern "C" int tn_dot(int dtype, const void* x, const void* y, const double* w, int64_t n,
+                      double* work, double* out, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int64_t g = (n + kThreads * 8 - 1) / (kThreads * 8);
+  if (g < 1) g = 1;
+  if (g > kMaxPartials) g = kMaxPartials;
+  if (dtype == TN_F64)
+    dot_partial<double><<<static_cast<unsigned>(g), kThreads, 0, s>>>(
+        static_cast<const double*>(x), static_cast<const double*>(y), w, n, work);
+  else if (dtype == TN_F32)
+    dot_partial<float><<<static_cast<unsigned>(g), kThreads, 0, s>>>(
+        static_cast<const float*>(x), static_cast<const float*>(y), w, n, work);
+  else
+    return tn::fail("tn_dot: unknown dtype");
+  dot_final<<<1, kThreads, 0, s>>>(work, static_cast<int>(g), out);
+  return tn::check_launch("tn_dot");
+}
+
+extern "C" int tn_axpby(int dtype, int64_t n, double alpha, const double* alpha_dev, int inv_alpha,
+                        const void* x, double beta, const double* beta_dev, int inv_beta, void* y,
+                        void* stream) {
+  if (n <= 0) return 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == TN_F64)
+    axpby_kernel<double><<<grid_for(n), kThreads, 0, s>>>(n, alpha, alpha_dev, inv_alpha,
+                                                          static_cast<const double*>(x), beta,
+                                                          beta_dev, inv_beta, static_cast<double*>(y));
+  else if (dtype == TN_F32)
+    axpby_kernel<float><<<grid_for(n), kThreads, 0, s>>>(n, alpha, alpha_dev, inv_alpha,
+                                                         static_cast<const float*>(x), beta,
+                                                         beta_dev, inv_beta, static_cast<float*>(y));
+  else
+    return tn::fail("tn_axpby: unknown dtype");
+  return tn::check_launch("tn_axpby");
+}
+
+extern "C" int tn_scale(int dtype, int64_t n, double sh, const double* sd, int inv, void* x,
+                        void* stream) {
+  if (n <= 0) return 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == TN_F64)
+    scale_kernel<double><<<grid_for(n), kThreads, 0, s>>>(n, sh, sd, inv, static_cast<double*>(x));
+  else if (dtype == TN_F32)
+    scale_kernel<float><<<grid_for(n), kThreads, 0, s>>>(n, sh, sd, inv, static_cast<float*>(x));
+  else
+    return tn::fail("tn_scale: unknown dtype");
+  return tn::check_launch("tn_scale");
+}
+
+extern "C" int tn_cast(int src_dtype, const void* src, int dst_dtype, void* dst, int64_t n,
+                       void* stream) {
+  if (n <= 0) return 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (src_dtype == TN_F64 && dst_dtype == TN_F32)
+    cast_kernel<double, float><<<grid_for(n), kThreads, 0, s>>>(static_cast<const double*>(src),
+                                                                static_cast<float*>(dst), n);
+  else if (src_dtype == TN_F32 && dst_dtype == TN_F64)
+    cast_kernel<float, double><<<grid_for(n), kThreads, 0, s>>>(static_cast<const float*>(src),
+                                                                static_cast<double*>(dst), n);
+  else
+    return tn::fail("tn_cast: unsupported dtype pair");
+  return tn::check_launch("tn_cast");
+}
+
+extern "C" int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t* jobs,
+                            int64_t njobs, const int64_t* in_slabs, const int64_t* out_slabs,
+                            const double* coef, const int64_t* tiles, int64_t ntiles,
+                            void* stream) {
+  (void)njobs;
+  if (ntiles <= 0) return 0;
+  if (ntiles > 0x7fffffffLL) return tn::fail("tn_mpo_apply: too many tiles");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const unsigned g = static_cast<unsigned>(ntiles);
+  if (dtype == TN_F64)
+    mpo_kernel<double><<<g, kThreads, 0, s>>>(static_cast<const double*>(in), static_cast<double*>(out),
+                                              jobs, in_slabs, out_slabs, coef, tiles);
+  else if (dtype == TN_F32)
+    mpo_kernel<float><<<g, kThreads, 0, s>>>(static_cast<const float*>(in), static_cast<float*>(out),
+                                             jobs, in_slabs, out_slabs, coef, tiles);
+  else
+    return tn::fail("tn_mpo_apply: unknown dtype");
+  return tn::check_launch("tn_mpo_apply");
+}

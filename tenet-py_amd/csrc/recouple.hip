@@ -1,0 +1,133 @@
+// Permute-and-recouple gather: dst_region = sum_t coeff_t * permute(src + off_t).
+//
+// One kernel moves every block of a repartition / transpose / to_matrices /
+// from_matrices step in a single launch, driven by a host-compiled job table (one job per
+// destination region; see tenet_amd.h for the word layout).  Each workgroup owns one tile
+// of one job:
+//  * linear tiles walk the destination in C order (coalesced stores); used when the
+//    destination-contiguous dim is also source-contiguous, or dims are tiny;
+//  * LDS tiles stage a 64x64 slab through LDS so that the loads run along the
+//    source-contiguous dim J and the stores along the destination-contiguous dim -- both
+//    coalesced -- with the batch over all remaining dims.
+// Multi-term jobs (SU(2) recoupling) accumulate all of a region's terms in registers
+// before the single store, so no atomics are needed and summation order is fixed.
+#include "tn_common.hpp"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLinPerThread = 8;
+constexpr int kLinTile = kThreads * kLinPerThread;
+constexpr int kT = 64;  // LDS tile edge
+
+template <typename T>
+__device__ __forceinline__ T gather_terms(const T* __restrict__ src, const int64_t* __restrict__ tsrc,
+                                          const double* __restrict__ tco, int64_t tb, int64_t te,
+                                          int64_t soff) {
+  T acc = T(0);
+  for (int64_t t = tb; t < te; ++t) acc += static_cast<T>(tco[t]) * src[tsrc[t] + soff];
+  return acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void recouple_kernel(const T* __restrict__ src,
+                                                            T* __restrict__ dst,
+                                                            const int64_t* __restrict__ jobs,
+                                                            const int64_t* __restrict__ tsrc,
+                                                            const double* __restrict__ tco,
+                                                            const int64_t* __restrict__ tiles) {
+  __shared__ T lds[kT][kT + 1];
+  const int64_t job_id = tiles[2 * blockIdx.x];
+  const int64_t tile = tiles[2 * blockIdx.x + 1];
+  const int64_t* jw = jobs + job_id * TN_JOB_WORDS;
+  const int64_t dst_off = jw[0], tb = jw[1], te = jw[2];
+  const int rank = static_cast<int>(jw[3]);
+  const int64_t* shape = jw + 4;
+  const int64_t* dstr = jw + 10;
+  const int64_t* sstr = jw + 16;
+  const int64_t mode = jw[22];
+  const int64_t nelem = jw[23];
+  const int tid = threadIdx.x;
+
+  if (mode < 0) {
+    // linear: element e of the job in destination C order
+    const int64_t base = tile * kLinTile;
+#pragma unroll 2
+    for (int r = 0; r < kLinPerThread; ++r) {
+      int64_t e = base + r * kThreads + tid;
+      if (e >= nelem) break;
+      int64_t so = 0, dof = 0;
+      uint32_t rem = static_cast<uint32_t>(e);
+      for (int d = rank - 1; d >= 0; --d) {
+        const uint32_t n = static_cast<uint32_t>(shape[d]);
+        const uint32_t q = rem / n;
+        const int64_t i = rem - q * n;
+        rem = q;
+        so += i * sstr[d];
+        dof += i * dstr[d];
+      }
+      dst[dst_off + dof] = gather_terms(src, tsrc, tco, tb, te, so);
+    }
+    return;
+  }
+
+  // LDS transpose tile over (I = rank-1, J = mode); remaining dims are batch.
+  const int I = rank - 1;
+  const int J = static_cast<int>(mode);
+  const int64_t nI = (shape[I] + kT - 1) / kT;
+  const int64_t nJ = (shape[J] + kT - 1) / kT;
+  const int64_t ti = tile % nI;
+  const int64_t tj = (tile / nI) % nJ;
+  uint32_t b = static_cast<uint32_t>(tile / (nI * nJ));
+  int64_t sbase = 0, dbase = 0;
+  for (int d = rank - 1; d >= 0; --d) {
+    if (d == I || d == J) continue;
+    const uint32_t n = static_cast<uint32_t>(shape[d]);
+    const uint32_t q = b / n;
+    const int64_t i = b - q * n;
+    b = q;
+    sbase += i * sstr[d];
+    dbase += i * dstr[d];
+  }
+  const int64_t i0 = ti * kT, j0 = tj * kT;
+  const int64_t ni = min<int64_t>(kT, shape[I] - i0);
+  const int64_t nj = min<int64_t>(kT, shape[J] - j0);
+  const int64_t sI = sstr[I], sJ = sstr[J], dJ = dstr[J], dI = dstr[I];
+  // load: threads run along J (source-contiguous)
+  for (int r = 0; r < kT * kT / kThreads; ++r) {
+    const int idx = r * kThreads + tid;
+    const int j = idx % kT, i = idx / kT;
+    if (i < ni && j < nj)
+      lds[i][j] = gather_terms(src, tsrc, tco, tb, te, sbase + (i0 + i) * sI + (j0 + j) * sJ);
+  }
+  __syncthreads();
+  // store: threads run along I (destination-contiguous)
+  for (int r = 0; r < kT * kT / kThreads; ++r) {
+    const int idx = r * kThreads + tid;
+    const int i = idx % kT, j = idx / kT;
+    if (i < ni && j < nj) dst[dst_off + dbase + (j0 + j) * dJ + (i0 + i) * dI] = lds[i][j];
+  }
+}
+
+}  // namespace
+
+extern "C" int tn_recouple(int dtype, const void* src, void* dst, const int64_t* jobs,
+                           const int64_t* term_src, const double* term_coeff,
+                           const int64_t* tiles, int64_t ntiles, void* stream) {
+  if (ntiles <= 0) return 0;
+  if (ntiles > 0x7fffffffLL) return tn::fail("tn_recouple: too many tiles");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dim3 grid(static_cast<unsigned>(ntiles)), block(kThreads);
+  if (dtype == TN_F64) {
+    recouple_kernel<double><<<grid, block, 0, s>>>(static_cast<const double*>(src),
+                                                   static_cast<double*>(dst), jobs, term_src,
+                                                   term_coeff, tiles);
+  } else if (dtype == TN_F32) {
+    recouple_kernel<float><<<grid, block, 0, s>>>(static_cast<const float*>(src),
+                                                  static_cast<float*>(dst), jobs, term_src,
+                                                  term_coeff, tiles);
+  } else {
+    return tn::fail("tn_recouple: unknown dtype");
+  }
+  return tn::check_launch("tn_recouple");
+}

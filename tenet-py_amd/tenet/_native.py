@@ -1,0 +1,91 @@
+"""ctypes binding of the gfx950 C-ABI library ``libtenet_amd.so`` (declared in
+``include/tenet_amd.h``).
+
+The library is loaded lazily on the first device call.  There is no fallback: if it is
+missing, or no HIP device is present, every compute entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+__all__ = ["lib", "check", "stream_ptr", "LIB_PATH", "TN_F64", "TN_F32", "PRECISIONS"]
+
+LIB_PATH = Path(__file__).resolve().parent / "libtenet_amd.so"
+
+TN_F64, TN_F32 = 0, 1
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+JOB_WORDS = 24
+MAX_RANK = 6
+PROB_WORDS = 8
+SEG_WORDS = 6
+MPO_JOB_WORDS = 8
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+_d = ctypes.c_double
+
+_SIGNATURES = {
+    "tn_last_error": (ctypes.c_char_p, []),
+    "tn_version": (_i32, []),
+    "tn_device_cus": (_i32, [ctypes.POINTER(_i32)]),
+    "tn_recouple": (_i32, [_i32, _p, _p, _p, _p, _p, _p, _i64, _p]),
+    "tn_gemm_grouped": (_i32, [_i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _p]),
+    "tn_dot": (_i32, [_i32, _p, _p, _p, _i64, _p, _p, _p]),
+    "tn_axpby": (_i32, [_i32, _i64, _d, _p, _i32, _p, _d, _p, _i32, _p, _p]),
+    "tn_scale": (_i32, [_i32, _i64, _d, _p, _i32, _p, _p]),
+    "tn_cast": (_i32, [_i32, _p, _i32, _p, _i64, _p]),
+    "tn_mpo_apply": (_i32, [_i32, _p, _p, _p, _i64, _p, _p, _p, _p, _i64, _p]),
+}
+
+_lib = None
+
+
+def load_library(path: Path | None = None) -> ctypes.CDLL:
+    """Open the shared library and declare every exported signature (no device needed)."""
+    path = Path(path or os.environ.get("TENET_AMD_LIB", LIB_PATH))
+    if not path.exists():
+        raise RuntimeError(
+            f"the gfx950 library {path} is not built; run `python tenet-py_amd/build.py` "
+            "(there is no CPU fallback for tenet's contraction engine)"
+        )
+    cdll = ctypes.CDLL(str(path))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(cdll, name)
+        fn.restype = res
+        fn.argtypes = args
+    return cdll
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises when it is missing or no GPU is visible."""
+    global _lib
+    if _lib is None:
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError(
+                "tenet's contraction engine runs only on a HIP device (MI355X / gfx950); "
+                "no GPU is visible to torch"
+            )
+        _lib = load_library()
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().tn_last_error().decode(errors="replace")
+        raise RuntimeError(f"tenet HIP kernel error: {msg}")
+
+
+def stream_ptr() -> int:
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
+def loaded() -> bool:
+    return _lib is not None

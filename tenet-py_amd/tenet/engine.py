@@ -1,0 +1,492 @@
+"""Compile host plans into device tables and run them through the gfx950 kernels.
+
+Two device programs cover all block movement and arithmetic of a contraction:
+
+* ``RecoupleProgram`` -- one ``tn_recouple`` launch: for every destination region
+  ``dst = sum_t coeff_t * permute(src_t)``.  Built from a (perm, terms) plan between two
+  flat layouts, or from arbitrary strided regions (sector matrices for the SVD).
+* ``GemmProgram`` -- one ``tn_gemm_grouped`` launch: for every output block,
+  ``C = sum_s A_s B_s`` over its K segments, operands read in place from flat buffers.
+
+``ContractionExec`` strings them together for ``tensordot``/``einsum``: operand
+repartitions that are pure block re-indexings (every U(1) bend, every interleaving
+change) cost nothing -- the GEMM reads the source blocks where they are -- and only a
+real in-block permutation or a multi-term recoupling launches ``tn_recouple``.  Every
+compiled program keeps its tables resident on the device and is cached on the frozen
+structures it was built from.
+"""
+
+from __future__ import annotations
+
+import os
+from collections import defaultdict
+from dataclasses import dataclass
+from functools import lru_cache
+
+import numpy as np
+import torch
+
+from tenet import _native
+from tenet._native import JOB_WORDS, MAX_RANK, PROB_WORDS, SEG_WORDS
+from tenet.plans import (
+    _compose_terms,
+    contraction_plan,
+    permutation_plan,
+    repartition_plan,
+)
+from tenet.structure import FusionBlockKey, TensorStructure
+
+__all__ = [
+    "RecoupleProgram",
+    "GemmProgram",
+    "ContractionExec",
+    "gemm_precision",
+    "set_gemm_precision",
+    "require_device",
+]
+
+_LIN_TILE = 2048
+_LDS_T = 64
+_GEMM_PRECISION = {"value": os.environ.get("TENET_GEMM_PRECISION", "fp32")}
+
+
+def set_gemm_precision(p: str) -> None:
+    """Arithmetic of GEMMs on float32 tensors: ``"fp32"`` or ``"bf16x3"``.  float64
+    tensors always use FP64 MFMA."""
+    if p not in ("fp32", "bf16x3"):
+        raise ValueError("gemm precision for float32 tensors is 'fp32' or 'bf16x3'")
+    _GEMM_PRECISION["value"] = p
+
+
+def gemm_precision(dtype: torch.dtype) -> str:
+    return "fp64" if dtype == torch.float64 else _GEMM_PRECISION["value"]
+
+
+def require_device(*tensors) -> None:
+    for t in tensors:
+        if not t.data.is_cuda:
+            raise RuntimeError(
+                "tenet's contraction engine runs on the HIP device only; move the tensor "
+                "with .to('cuda') (there is no CPU fallback)"
+            )
+    _native.lib()
+
+
+def _dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float64:
+        return _native.TN_F64
+    if dtype == torch.float32:
+        return _native.TN_F32
+    raise TypeError(f"unsupported dtype {dtype}")
+
+
+class _DeviceCache:
+    """Lazily uploaded copies of host tables, one per device."""
+
+    def __init__(self, **arrays) -> None:
+        self.host = arrays
+        self._dev: dict = {}
+
+    def get(self, device: torch.device) -> dict:
+        key = (device.type, device.index)
+        d = self._dev.get(key)
+        if d is None:
+            d = {k: torch.from_numpy(np.ascontiguousarray(v)).to(device) for k, v in self.host.items()}
+            self._dev[key] = d
+        return d
+
+
+# ----------------------------------------------------------------------------- recouple
+
+
+def _normalize(shape, dstr, sstr):
+    """Drop unit dims and merge dims that are contiguous in both source and destination."""
+    dims = [(n, d, s) for n, d, s in zip(shape, dstr, sstr) if n != 1]
+    if not dims:
+        return [1], [1], [1]
+    merged = [list(dims[0])]
+    for n, d, s in dims[1:]:
+        pn, pd, ps = merged[-1]
+        if pd == d * n and ps == s * n:
+            merged[-1] = [pn * n, d, s]
+        else:
+            merged.append([n, d, s])
+    return [m[0] for m in merged], [m[1] for m in merged], [m[2] for m in merged]
+
+
+class RecoupleProgram:
+    """A compiled ``tn_recouple`` launch.
+
+    ``regions`` is a list of ``(dst_off, shape, dst_strides, src_strides, [(src_off,
+    coeff), ...])``; dims are given in destination order.
+    """
+
+    def __init__(self, regions, src_size: int, dst_size: int) -> None:
+        n = len(regions)
+        jobs = np.zeros((n, JOB_WORDS), dtype=np.int64)
+        tsrc, tco, tiles = [], [], []
+        tb = 0
+        self.elements = 0
+        self.moved_terms = 0
+        for j, (doff, shape, dstr, sstr, terms) in enumerate(regions):
+            shp, ds, ss = _normalize(shape, dstr, sstr)
+            if len(shp) > MAX_RANK:
+                raise ValueError(f"recouple region of rank {len(shp)} exceeds {MAX_RANK}")
+            r = len(shp)
+            nelem = int(np.prod(shp))
+            mode = -1
+            if ss[-1] != 1 and ds[-1] == 1 and shp[-1] >= 16:
+                cands = [k for k in range(r - 1) if ss[k] == 1 and shp[k] >= 16]
+                if cands:
+                    mode = cands[0]
+            jobs[j, 0] = doff
+            jobs[j, 1] = tb
+            jobs[j, 2] = tb + len(terms)
+            jobs[j, 3] = r
+            jobs[j, 4 : 4 + r] = shp
+            jobs[j, 10 : 10 + r] = ds
+            jobs[j, 16 : 16 + r] = ss
+            jobs[j, 22] = mode
+            jobs[j, 23] = nelem
+            for so, c in terms:
+                tsrc.append(so)
+                tco.append(float(np.real(c)))
+            tb += len(terms)
+            if mode < 0:
+                nt = -(-nelem // _LIN_TILE)
+            else:
+                batch = nelem // (shp[-1] * shp[mode])
+                nt = (-(-shp[-1] // _LDS_T)) * (-(-shp[mode] // _LDS_T)) * batch
+            tiles.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
+            self.elements += nelem
+            self.moved_terms += nelem * len(terms)
+            # every access stays in bounds: checked here once, on the host
+            hi_dst = doff + sum((a - 1) * b for a, b in zip(shp, ds))
+            if doff < 0 or hi_dst >= dst_size:
+                raise ValueError("recouple region writes outside its destination buffer")
+            span = sum((a - 1) * b for a, b in zip(shp, ss))
+            for so, _ in terms:
+                if so < 0 or so + span >= src_size:
+                    raise ValueError("recouple term reads outside its source buffer")
+        self.njobs = n
+        tiles_arr = np.concatenate(tiles) if tiles else np.zeros((0, 2), np.int64)
+        self.ntiles = len(tiles_arr)
+        self.src_size, self.dst_size = src_size, dst_size
+        self._tables = _DeviceCache(
+            jobs=jobs.reshape(-1),
+            tsrc=np.asarray(tsrc, dtype=np.int64),
+            tco=np.asarray(tco, dtype=np.float64),
+            tiles=tiles_arr.reshape(-1),
+        )
+
+    def bytes_moved(self, itemsize: int) -> int:
+        """Algorithmic HBM bytes: each term's source read once, each destination written once."""
+        return (self.moved_terms + self.elements) * itemsize
+
+    def run(self, src: torch.Tensor, dst: torch.Tensor) -> None:
+        if self.ntiles == 0:
+            return
+        if src.dtype != dst.dtype:
+            raise TypeError("recouple: source and destination dtypes differ")
+        if src.numel() < self.src_size or dst.numel() < self.dst_size:
+            raise ValueError("recouple: buffers smaller than the compiled plan")
+        t = self._tables.get(dst.device)
+        _native.check(
+            _native.lib().tn_recouple(
+                _dtype_code(dst.dtype),
+                src.data_ptr(),
+                dst.data_ptr(),
+                t["jobs"].data_ptr(),
+                t["tsrc"].data_ptr(),
+                t["tco"].data_ptr(),
+                t["tiles"].data_ptr(),
+                self.ntiles,
+                _native.stream_ptr(),
+            )
+        )
+
+
+def regions_for_plan(S: TensorStructure, D: TensorStructure, perm, terms):
+    """Destination regions of a (perm, terms) block map between two flat layouts."""
+    by_dst = defaultdict(list)
+    for i, j, c in terms:
+        by_dst[j].append((i, c))
+    if len(by_dst) != D.num_blocks:
+        raise ValueError(
+            f"the plan fills {len(by_dst)} of {D.num_blocks} target blocks; the provider's "
+            "coefficients dropped terms"
+        )
+    sl, dl = S.layout, D.layout
+    regions = []
+    for j in range(D.num_blocks):
+        lst = by_dst[j]
+        i0 = lst[0][0]
+        sstr = tuple(int(sl.strides[i0][p]) for p in perm)
+        regions.append(
+            (
+                int(dl.offsets[j]),
+                tuple(int(x) for x in dl.shapes[j]),
+                tuple(int(x) for x in dl.strides[j]),
+                sstr,
+                [(int(sl.offsets[i]), c) for i, c in lst],
+            )
+        )
+    return regions
+
+
+@lru_cache(maxsize=4096)
+def recouple_program(S: TensorStructure, D: TensorStructure, perm: tuple, terms: tuple):
+    return RecoupleProgram(regions_for_plan(S, D, perm, terms), S.size, D.size)
+
+
+def inplace_map(S: TensorStructure, D: TensorStructure, perm, terms):
+    """Offsets of D's blocks inside S's buffer when the map is a pure re-indexing
+    (identity permutation, one-to-one, coefficient 1); else None."""
+    if tuple(perm) != tuple(range(len(perm))):
+        return None
+    if len(terms) != D.num_blocks or len(terms) != S.num_blocks:
+        return None
+    offs = np.empty(D.num_blocks, dtype=np.int64)
+    seen = np.zeros(D.num_blocks, dtype=bool)
+    sl = S.layout.offsets
+    for i, j, c in terms:
+        if c != 1 or seen[j]:
+            return None
+        seen[j] = True
+        offs[j] = sl[i]
+    return offs if seen.all() else None
+
+
+# ----------------------------------------------------------------------------- gemm
+
+_TILES = ((128, 128), (128, 64), (64, 128), (64, 64), (32, 32))
+
+
+def choose_tile(Ms, Ns, Ks) -> tuple:
+    """Pick the workgroup tile from the problem-size histogram: the largest tile whose
+    padded work stays within 15% of the best achievable utilisation."""
+    Ms, Ns, Ks = (np.asarray(x, dtype=np.float64) for x in (Ms, Ns, Ks))
+    useful = float((Ms * Ns * Ks).sum()) or 1.0
+    util = {}
+    for bm, bn in _TILES:
+        padded = float((np.ceil(Ms / bm) * bm * np.ceil(Ns / bn) * bn * Ks).sum()) or 1.0
+        util[(bm, bn)] = useful / padded
+    best = max(util.values())
+    for cfg in _TILES:
+        if util[cfg] >= 0.85 * best:
+            return cfg
+    return (64, 64)
+
+
+class GemmProgram:
+    """A compiled ``tn_gemm_grouped`` launch over a list of block problems.
+
+    ``problems``: list of ``(M, N, c_off, ldc, [(a_off, b_off, K, lda, ldb), ...])``.
+    """
+
+    def __init__(self, problems, a_size, b_size, c_size, trans_a=False, trans_b=False,
+                 tile=None) -> None:
+        self.trans_a, self.trans_b = bool(trans_a), bool(trans_b)
+        P = len(problems)
+        probs = np.zeros((P, PROB_WORDS), dtype=np.int64)
+        segs = []
+        Ms, Ns, Ks = [], [], []
+        self.flops = 0
+        for p, (M, N, c_off, ldc, sl) in enumerate(problems):
+            probs[p, :6] = (M, N, c_off, ldc, len(segs), len(segs) + len(sl))
+            if c_off < 0 or c_off + (M - 1) * ldc + N > c_size:
+                raise ValueError("gemm problem writes outside the output buffer")
+            ktot = 0
+            for a_off, b_off, K, lda, ldb in sl:
+                a_hi = a_off + ((K - 1) * lda + M if self.trans_a else (M - 1) * lda + K)
+                b_hi = b_off + ((N - 1) * ldb + K if self.trans_b else (K - 1) * ldb + N)
+                if a_off < 0 or b_off < 0 or a_hi > a_size or b_hi > b_size:
+                    raise ValueError("gemm segment reads outside its operand buffer")
+                segs.append((a_off, b_off, K, lda, ldb, 0))
+                ktot += K
+            self.flops += 2 * M * N * ktot
+            Ms.append(M)
+            Ns.append(N)
+            Ks.append(max(ktot, 1))
+        self.Ms, self.Ns, self.Ks = Ms, Ns, Ks
+        self.a_size, self.b_size, self.c_size = a_size, b_size, c_size
+        self.nproblems = P
+        self._probs = probs
+        self._segs = np.asarray(segs, dtype=np.int64).reshape(-1, SEG_WORDS)
+        self._by_tile: dict = {}
+        self.tile = tuple(tile) if tile else (choose_tile(Ms, Ns, Ks) if P else (64, 64))
+
+    def _tables(self, tile, device):
+        d = self._by_tile.get(tile)
+        if d is None:
+            bm, bn = tile
+            rows = []
+            for p, (M, N) in enumerate(zip(self.Ms, self.Ns)):
+                tm, tn = -(-M // bm), -(-N // bn)
+                if tm * tn == 0:
+                    continue
+                ii, jj = np.meshgrid(np.arange(tm), np.arange(tn), indexing="ij")
+                rows.append(np.stack([np.full(tm * tn, p), ii.ravel(), jj.ravel(),
+                                      np.zeros(tm * tn)], 1))
+            tiles = np.concatenate(rows).astype(np.int32) if rows else np.zeros((0, 4), np.int32)
+            d = _DeviceCache(probs=self._probs.reshape(-1), segs=self._segs.reshape(-1),
+                             tiles=tiles.reshape(-1))
+            d.ntiles = len(tiles)
+            self._by_tile[tile] = d
+        return d.get(device), d.ntiles
+
+    def run(self, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, precision: str | None = None,
+            tile=None, grid: int = 0) -> None:
+        if self.nproblems == 0:
+            return
+        if not (A.dtype == B.dtype == C.dtype):
+            raise TypeError("gemm: operand dtypes differ")
+        if A.numel() < self.a_size or B.numel() < self.b_size or C.numel() < self.c_size:
+            raise ValueError("gemm: buffers smaller than the compiled plan")
+        prec = precision or gemm_precision(C.dtype)
+        tile = tuple(tile) if tile else self.tile
+        t, ntiles = self._tables(tile, C.device)
+        _native.check(
+            _native.lib().tn_gemm_grouped(
+                _native.PRECISIONS[prec],
+                _dtype_code(C.dtype),
+                int(self.trans_a),
+                int(self.trans_b),
+                A.data_ptr(),
+                B.data_ptr(),
+                C.data_ptr(),
+                t["probs"].data_ptr(),
+                t["segs"].data_ptr(),
+                t["tiles"].data_ptr(),
+                ntiles,
+                tile[0],
+                tile[1],
+                grid,
+                _native.stream_ptr(),
+            )
+        )
+
+
+# ----------------------------------------------------------------------------- contraction
+
+
+def compose_plans(first, second):
+    """(perm, terms) of applying plan ``first`` then plan ``second``."""
+    p1, t1 = first
+    p2, t2 = second
+    return tuple(p1[i] for i in p2), _compose_terms(t1, t2)
+
+
+@dataclass
+class _Operand:
+    """How a GEMM operand is reached: in place (offsets into the caller's buffer) or via a
+    recouple program into a scratch buffer of ``structure``'s layout."""
+
+    structure: TensorStructure
+    offsets: np.ndarray
+    program: RecoupleProgram | None
+
+
+def _operand(S: TensorStructure, plan) -> _Operand:
+    D = plan.new_structure
+    offs = inplace_map(S, D, plan.perm, plan.terms)
+    if offs is not None:
+        return _Operand(D, offs, None)
+    prog = recouple_program(S, D, tuple(plan.perm), tuple(plan.terms))
+    return _Operand(D, D.layout.offsets, prog)
+
+
+class ContractionExec:
+    """Compiled ``tensordot(a, b, axes)`` followed by an optional output transpose."""
+
+    def __init__(self, sa: TensorStructure, sb: TensorStructure, axes: tuple, out_axes=None):
+        cp = contraction_plan(sa, sb, axes)
+        rp_a = repartition_plan(sa, cp.a_outputs, cp.a_inputs)
+        rp_b = repartition_plan(sb, cp.b_outputs, cp.b_inputs)
+        self.a = _operand(sa, rp_a)
+        self.b = _operand(sb, rp_b)
+        Ap, Bp = self.a.structure, self.b.structure
+        n_out_a = len(cp.a_outputs)
+        n_out_b = len(cp.b_outputs)
+        Cp = TensorStructure((*(Ap.legs[i] for i in Ap.out_axes), *(Bp.legs[i] for i in Bp.in_axes)))
+        # C' -> final: restore sides, tensordot order, then the caller's transpose
+        r = repartition_plan(Cp, cp.restore_outputs, cp.restore_inputs)
+        plan = (r.perm, r.terms)
+        cur = r.new_structure
+        t = permutation_plan(cur, cp.final_transpose)
+        plan = compose_plans(plan, (t.axes, t.terms))
+        cur = t.new_structure
+        if out_axes is not None and tuple(out_axes) != tuple(range(cur.ndim)):
+            t = permutation_plan(cur, tuple(out_axes))
+            plan = compose_plans(plan, (t.axes, t.terms))
+            cur = t.new_structure
+        self.out_structure = cur
+        self.c_structure = Cp
+        c_offs = inplace_map(Cp, cur, plan[0], plan[1])
+        if c_offs is None:
+            self.c_program = recouple_program(Cp, cur, tuple(plan[0]), tuple(plan[1]))
+            c_offs = Cp.layout.offsets
+            c_size = Cp.size
+        else:
+            self.c_program = None
+            c_size = cur.size
+        # ---- block problems of the composition C' = A' o B'
+        shapes_a, shapes_b = Ap.shapes, Bp.shapes
+        shapes_c = Cp.shapes
+        b_index = {}
+        for j, key in enumerate(Bp.block_order):
+            b_index.setdefault(key.coupled, {}).setdefault(key.output_tree, []).append((key.input_tree, j))
+        a_rows = {}
+        for j, key in enumerate(Ap.block_order):
+            a_rows.setdefault((key.coupled, key.output_tree), []).append((key.input_tree, j))
+        b_lookup = {}
+        for j, key in enumerate(Bp.block_order):
+            b_lookup[(key.output_tree, key.input_tree)] = j
+        problems = []
+        for i, key in enumerate(Cp.block_order):
+            shp = shapes_c[i]
+            M = int(np.prod(shp[:n_out_a], dtype=np.int64)) if n_out_a else 1
+            N = int(np.prod(shp[n_out_a:], dtype=np.int64)) if len(shp) > n_out_a else 1
+            segs = []
+            for kt, ja in a_rows.get((key.coupled, key.output_tree), ()):
+                jb = b_lookup.get((kt, key.input_tree))
+                if jb is None:
+                    continue
+                K = int(np.prod(shapes_a[ja][n_out_a:], dtype=np.int64))
+                segs.append((int(self.a.offsets[ja]), int(self.b.offsets[jb]), K, K, N))
+            problems.append((M, N, int(c_offs[i]), N, segs))
+        a_size = sa.size if self.a.program is None else Ap.size
+        b_size = sb.size if self.b.program is None else Bp.size
+        self.gemm = GemmProgram(problems, a_size, b_size, c_size)
+        self.flops = self.gemm.flops
+        _ = n_out_b
+
+    def run(self, a_data: torch.Tensor, b_data: torch.Tensor, precision=None, tile=None,
+            out: torch.Tensor | None = None) -> torch.Tensor:
+        dtype, dev = a_data.dtype, a_data.device
+        if b_data.dtype != dtype:
+            raise TypeError("tensordot: operands have different dtypes")
+        if self.a.program is not None:
+            ap = torch.empty(self.a.structure.size, dtype=dtype, device=dev)
+            self.a.program.run(a_data, ap)
+        else:
+            ap = a_data
+        if self.b.program is not None:
+            bp = torch.empty(self.b.structure.size, dtype=dtype, device=dev)
+            self.b.program.run(b_data, bp)
+        else:
+            bp = b_data
+        if out is None:
+            out = torch.empty(self.out_structure.size, dtype=dtype, device=dev)
+        if self.c_program is None:
+            self.gemm.run(ap, bp, out, precision, tile)
+        else:
+            cp = torch.empty(self.c_structure.size, dtype=dtype, device=dev)
+            self.gemm.run(ap, bp, cp, precision, tile)
+            self.c_program.run(cp, out)
+        return out
+
+
+@lru_cache(maxsize=2048)
+def contraction_exec(sa: TensorStructure, sb: TensorStructure, axes: tuple, out_axes=None):
+    return ContractionExec(sa, sb, axes, out_axes)

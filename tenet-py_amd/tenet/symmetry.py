@@ -1,0 +1,358 @@
+"""Symmetry providers: sector labels, fusion rules and recoupling coefficients.
+
+Sectors are frozen, ordered labels; everything a symmetry *does* lives on its provider.
+Three providers cover the contraction engine's workloads:
+
+* ``Trivial``  -- one sector, every coefficient 1 (reference ``symmetry/base.py:1193-1262``).
+* ``U1``       -- integer charges, ``dual(q) = -q``, F = R = B = FS = 1
+  (reference ``symmetry/u1.py:42-126``).
+* ``SU2``      -- labels are ``2j``; triangle fusion, ``qdim = 2j+1``, F from the Racah
+  6j formula in the TensorKit gauge, ``R = (-1)^(ja+jb-jc)``, ``B`` derived from F and
+  ``chi = (-1)^(2j)`` (reference ``symmetry/su2.py:150-252``, ``_sun_coeff.py:75-117``).
+
+Coefficients are host-side plan data: they enter only when a plan is compiled, never per
+block element on the device.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from functools import cache
+from math import sqrt
+from typing import TYPE_CHECKING
+
+from tenet import _su2coeff
+
+if TYPE_CHECKING:
+    from tenet.fusion_tree import FusionTree
+    from tenet.structure import FusionBlockKey
+
+__all__ = [
+    "CapabilityError",
+    "StructureChangingError",
+    "Sector",
+    "TrivialSector",
+    "U1Sector",
+    "SU2Sector",
+    "Trivial",
+    "U1",
+    "SU2",
+]
+
+
+class CapabilityError(TypeError):
+    """A provider lacks a capability an operation requires (reference ``base.py:700``)."""
+
+
+class StructureChangingError(TypeError):
+    """An output structure depends on block values inside a traced region (``base.py:704``)."""
+
+
+@dataclass(frozen=True, slots=True, order=True)
+class Sector:
+    """Marker base for sector labels; subclasses are frozen, slotted and ordered."""
+
+
+@dataclass(frozen=True, slots=True, order=True)
+class TrivialSector(Sector):
+    """The single sector of the trivial symmetry."""
+
+
+@dataclass(frozen=True, slots=True, order=True)
+class U1Sector(Sector):
+    """A U(1) irrep labelled by its integer charge; negation is duality."""
+
+    charge: int
+
+
+@dataclass(frozen=True, slots=True, order=True)
+class SU2Sector(Sector):
+    """An SU(2) irrep labelled by ``two_j`` (so labels stay integers)."""
+
+    two_j: int
+
+    def __post_init__(self) -> None:
+        if not isinstance(self.two_j, int) or isinstance(self.two_j, bool):
+            raise TypeError(f"two_j must be an int, got {self.two_j!r}")
+        if self.two_j < 0:
+            raise ValueError(f"two_j must be non-negative, got {self.two_j}")
+
+
+# --------------------------------------------------------------------------------------
+# Unique-fusion helpers (Trivial, U1): one term, coefficient exactly 1.
+# Reference: ``permute_unique_tree`` and ``bend_unique`` in ``symmetry/base.py:778-1074``.
+# The spine of the new tree is recomputed from the new uncoupled labels.
+# --------------------------------------------------------------------------------------
+
+
+def _only_tree(provider, uncoupled, coupled) -> "FusionTree":
+    from tenet.fusion_tree import fusion_trees
+
+    trees = fusion_trees(provider, uncoupled, coupled)
+    if len(trees) != 1:
+        raise CapabilityError(
+            f"{provider.name}: {uncoupled} -> {coupled!r} has {len(trees)} trees; a "
+            "unique-fusion recoupling needs exactly one"
+        )
+    return trees[0]
+
+
+def _permute_unique(provider, tree, perm):
+    uncoupled = tuple(tree.uncoupled[i] for i in perm)
+    return ((_only_tree(provider, uncoupled, tree.coupled), 1.0),)
+
+
+def _bend_unique(provider, key, right: bool):
+    from tenet.fusion_tree import coupled_sectors
+    from tenet.structure import FusionBlockKey
+
+    src = key.output_tree if right else key.input_tree
+    dst = key.input_tree if right else key.output_tree
+    if src.rank == 0:
+        raise ValueError(f"{provider.name}: cannot bend from an empty tree of {key}")
+    moved, remaining = src.uncoupled[-1], src.uncoupled[:-1]
+    coupled = coupled_sectors(provider, remaining)
+    if len(coupled) != 1:
+        raise CapabilityError(f"{provider.name}: {remaining} does not fuse uniquely")
+    new_src = _only_tree(provider, remaining, coupled[0])
+    new_dst = _only_tree(provider, (*dst.uncoupled, provider.dual(moved)), coupled[0])
+    new_key = FusionBlockKey(new_src, new_dst) if right else FusionBlockKey(new_dst, new_src)
+    return ((new_key, 1.0),)
+
+
+@dataclass(frozen=True, slots=True)
+class TrivialProvider:
+    """The trivial symmetry: one sector, every coefficient 1."""
+
+    name: str = "Trivial"
+
+    @property
+    def unit(self) -> TrivialSector:
+        return TrivialSector()
+
+    def dual(self, a):
+        return a
+
+    def fusion(self, a, b):
+        return (TrivialSector(),)
+
+    def n_symbol(self, a, b, c) -> int:
+        return 1
+
+    def qdim(self, a) -> float:
+        return 1.0
+
+    def frobenius_schur(self, a) -> float:
+        return 1.0
+
+    def permute_tree(self, tree, perm):
+        return _permute_unique(self, tree, perm)
+
+    def bend_right(self, key, *, dual: bool):
+        return _bend_unique(self, key, True)
+
+    def bend_left(self, key, *, dual: bool):
+        return _bend_unique(self, key, False)
+
+    @property
+    def unique_fusion(self) -> bool:
+        return True
+
+
+@dataclass(frozen=True, slots=True)
+class U1Provider:
+    """U(1): abelian, multiplicity free, every recoupling coefficient exactly 1.
+
+    Reference ``symmetry/u1.py:42-126``.
+    """
+
+    name: str = "U1"
+
+    @property
+    def unit(self) -> U1Sector:
+        return U1Sector(0)
+
+    def dual(self, a: U1Sector) -> U1Sector:
+        return U1Sector(-a.charge)
+
+    def fusion(self, a: U1Sector, b: U1Sector) -> tuple[U1Sector, ...]:
+        return (U1Sector(a.charge + b.charge),)
+
+    def n_symbol(self, a, b, c) -> int:
+        return int(c.charge == a.charge + b.charge)
+
+    def qdim(self, a) -> float:
+        return 1.0
+
+    def frobenius_schur(self, a) -> float:
+        return 1.0
+
+    def permute_tree(self, tree, perm):
+        return _permute_unique(self, tree, perm)
+
+    def bend_right(self, key, *, dual: bool):
+        return _bend_unique(self, key, True)
+
+    def bend_left(self, key, *, dual: bool):
+        return _bend_unique(self, key, False)
+
+    @property
+    def unique_fusion(self) -> bool:
+        return True
+
+
+def _triangle(a: int, b: int, c: int) -> bool:
+    return abs(a - b) <= c <= a + b and (a + b + c) % 2 == 0
+
+
+@dataclass(frozen=True, slots=True)
+class SU2Provider:
+    """SU(2) with sectors labelled by ``2j``.
+
+    Fusion ``a x b = |a-b|, |a-b|+2, ..., a+b``; ``qdim = 2j+1``; self-dual labels.
+    F-symbols come from the host Racah/6j routine in the TensorKit gauge
+    (``tenet._su2coeff``), R from the exchange phase, B and the Frobenius-Schur
+    indicator derived from F exactly as reference ``_sun_coeff.py:90-106`` defines them.
+    Braids follow reference ``permute_braided_tree``/``_artin_braid``
+    (``symmetry/base.py:828-996``) and bends ``bend_braided`` (``base.py:1077-1190``).
+    """
+
+    name: str = "SU2"
+
+    @property
+    def unit(self) -> SU2Sector:
+        return SU2Sector(0)
+
+    def dual(self, a: SU2Sector) -> SU2Sector:
+        return a
+
+    def fusion(self, a: SU2Sector, b: SU2Sector) -> tuple[SU2Sector, ...]:
+        return tuple(SU2Sector(t) for t in range(abs(a.two_j - b.two_j), a.two_j + b.two_j + 1, 2))
+
+    def n_symbol(self, a, b, c) -> int:
+        return int(_triangle(a.two_j, b.two_j, c.two_j))
+
+    def qdim(self, a: SU2Sector) -> float:
+        return float(a.two_j + 1)
+
+    def f_symbol(self, a, b, c, d, e, f) -> float:
+        return _su2coeff.f_symbol(a.two_j, b.two_j, c.two_j, d.two_j, e.two_j, f.two_j)
+
+    def r_symbol(self, a, b, c) -> float:
+        return _su2coeff.r_symbol(a.two_j, b.two_j, c.two_j)
+
+    def b_symbol(self, a, b, c) -> float:
+        return _su2coeff.b_symbol(a.two_j, b.two_j, c.two_j)
+
+    def frobenius_schur(self, a: SU2Sector) -> float:
+        return _su2coeff.frobenius_schur(a.two_j)
+
+    def permute_tree(self, tree, perm):
+        return _permute_braided(self, tree, tuple(perm))
+
+    def bend_right(self, key, *, dual: bool):
+        return _bend_braided(self, key, True, dual)
+
+    def bend_left(self, key, *, dual: bool):
+        return _bend_braided(self, key, False, dual)
+
+    @property
+    def unique_fusion(self) -> bool:
+        return False
+
+
+# --------------------------------------------------------------------------------------
+# Braided recoupling for multiplicity-free providers with real F and R (SU(2)).
+# --------------------------------------------------------------------------------------
+
+
+@cache
+def _permute_braided(provider, tree, perm):
+    """Expand ``tree`` with uncoupled lines permuted by ``perm`` into a tree sum.
+
+    ``perm[j]`` is the old position that becomes position ``j``.  The permutation is
+    bubble-sorted into adjacent swaps applied in order (reference ``base.py:884-895``);
+    each swap is one Artin generator.  Returns ``((tree, coeff), ...)`` in first-seen
+    order of the accumulation.
+    """
+    terms: dict = {tree: 1.0}
+    current = list(range(tree.rank))
+    for j, target in enumerate(perm):
+        for i in range(current.index(target) - 1, j - 1, -1):
+            current[i], current[i + 1] = current[i + 1], current[i]
+            nxt: dict = {}
+            for t, c in terms.items():
+                for braided, cb in _swap(provider, t, i):
+                    nxt[braided] = nxt.get(braided, 0.0) + c * cb
+            terms = nxt
+    return tuple(terms.items())
+
+
+@cache
+def _swap(provider, tree, i: int):
+    """Exchange uncoupled lines ``i`` and ``i+1`` of a left-associated tree.
+
+    ``i == 0``: one R-move on the first vertex.  ``i >= 1``: with ``a = e_{i-1}``,
+    ``b = u_i``, ``c = u_{i+1}``, ``f = e_{i+1}`` and old inner line ``e``,
+    ``coeff(e -> e') = sum_d F^{abc}_f[e,d] R^{bc}_d F^{acb}_f[e',d]`` (real gauge).
+    Reference ``_artin_braid``, ``symmetry/base.py:898-996``.
+    """
+    from tenet.fusion_tree import FusionTree
+
+    u = tree.uncoupled
+    spine = tree.lines()
+    new_u = (*u[:i], u[i + 1], u[i], *u[i + 2 :])
+    if i == 0:
+        swapped = FusionTree(new_u, tree.inner, tree.multiplicities, tree.coupled)
+        return ((swapped, provider.r_symbol(u[0], u[1], spine[1])),)
+    a, b, c, e, f = spine[i - 1], u[i], u[i + 1], spine[i], spine[i + 1]
+    out = []
+    for new_e in provider.fusion(a, c):
+        if not provider.n_symbol(new_e, b, f):
+            continue
+        inner = (*tree.inner[: i - 1], new_e, *tree.inner[i:])
+        coeff = 0.0
+        for d in provider.fusion(b, c):
+            coeff += (
+                provider.f_symbol(a, b, c, f, e, d)
+                * provider.r_symbol(b, c, d)
+                * provider.f_symbol(a, c, b, f, new_e, d)
+            )
+        out.append((FusionTree(new_u, inner, tree.multiplicities, tree.coupled), coeff))
+    return tuple(out)
+
+
+def _bend_braided(provider, key, right: bool, dual: bool):
+    """Move the last line of one tree to the end of the other.
+
+    Source vertex ``a x b -> c`` (``a`` the unit for a rank-1 source); the coefficient is
+    ``sqrt(qdim(c)/qdim(a)) * B(a,b,c)``, times ``chi(dual(b))`` when the moved leg was
+    already dual; ``bend_left`` reads the same expression off the input tree (its
+    conjugate, equal for a real gauge).  Reference ``bend_braided``, ``base.py:1077-1190``.
+    """
+    from tenet.fusion_tree import FusionTree
+    from tenet.structure import FusionBlockKey
+
+    src = key.output_tree if right else key.input_tree
+    dst = key.input_tree if right else key.output_tree
+    if src.rank == 0:
+        raise ValueError(f"{provider.name}: cannot bend from an empty tree of {key}")
+    b, c = src.uncoupled[-1], src.coupled
+    a = src.lines()[-2] if src.rank >= 2 else provider.unit
+    new_src = FusionTree(src.uncoupled[:-1], src.inner[:-1], src.multiplicities[:-1], a)
+    scale = sqrt(provider.qdim(c) / provider.qdim(a))
+    if dual:
+        scale *= provider.frobenius_schur(provider.dual(b))
+    new_dst = FusionTree(
+        (*dst.uncoupled, provider.dual(b)),
+        dst.lines()[1:],
+        (*dst.multiplicities, 0) if dst.rank else (),
+        a,
+    )
+    new_key = FusionBlockKey(new_src, new_dst) if right else FusionBlockKey(new_dst, new_src)
+    return ((new_key, scale * provider.b_symbol(a, b, c)),)
+
+
+Trivial = TrivialProvider()
+U1 = U1Provider()
+SU2 = SU2Provider()
