@@ -1,0 +1,220 @@
+"""Benchmark: two-site H_eff matvecs/s of the U(1) Heisenberg chain (L=64, D=2048 at the
+centre bond, Gaussian sector profile sigma=6, MPO bond 5, physical dim 2) on one MI355X.
+
+One step = one H_eff matvec on the centre bond (all four contractions), inputs resident
+in HBM.  Arms timed on the same operands:
+
+* ``fp32``       -- the engine with FP32 MFMA GEMMs (the headline ``value``)
+* ``bf16x3``     -- the engine with split-BF16x3 MFMA GEMMs (FP32 accumulation)
+* ``fp64``       -- the engine with FP64 MFMA GEMMs
+* ``fp64_blocks``-- the plain FP64 per-block GPU path (the reference's lowering run with
+  one torch call per block/sector)
+* ``cpu_baseline`` -- the oracle's NumPy FP64 restatement on the host cores
+
+Energy check: a restarted FP64-Krylov Lanczos on this H_eff with FP32 (and BF16x3)
+matvecs plus a closing FP64 Rayleigh quotient, against the same solve with FP64
+matvecs; ``dE`` is the relative difference.
+
+Usage: ``python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]``; prints ONE
+JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "tenet-py_amd"))
+
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16x3": 2500.0}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_arm(fn, steps: int, warmup: int):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def restarted_lanczos(op, aa, refine=None, restarts: int = 6, ncv: int = 4):
+    from tenet.network import lanczos
+
+    v = aa
+    e = None
+    for _ in range(restarts):
+        e, v = lanczos(op, v, ncv=ncv, refine=refine)
+    return e
+
+
+def cpu_baseline(fl, w1, w2, fr, aa, budget_s: float = 15.0):
+    """The oracle's NumPy FP64 H_eff on the host cores; bounded sample of whole matvecs."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT))
+    from oracle import u1_network as N
+    from tests.helpers import to_oracle
+
+    cores = len(os.sched_getaffinity(0))
+    ofl, ow1, ow2, ofr, oaa = (to_oracle(t) for t in (fl, w1, w2, fr, aa))
+    t0 = time.perf_counter()
+    out = N.heff2(ofl, ow1, ow2, ofr, oaa)  # warm-up, also the full-size parity reference
+    first = time.perf_counter() - t0
+    reps = max(1, min(20, int(budget_s / max(first, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        N.heff2(ofl, ow1, ow2, ofr, oaa)
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": 1.0 / dt, "unit": "matvecs/s", "cores": cores, "kind": "port",
+            "sample": f"{reps} timed H_eff matvecs after 1 warm-up, NumPy float64, BLAS threads = {cores}"}, out.flat()
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--sites", type=int, default=64)
+    ap.add_argument("--bond", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-blocks", action="store_true")
+    ap.add_argument("--arms", default="fp32,bf16x3,fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
+              "(the H_eff of one bond runs on one device; see the sharding note in DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import tenet
+    from tenet.baseline_blocks import BlockTensor, heff2 as blocks_heff2
+    from tenet.network import Heff2
+    from tenet.network.workloads import heff2_problem
+
+    dev = torch.device("cuda")
+    log(f"building the L={args.sites} D={args.bond} problem")
+    t0 = time.perf_counter()
+    fl, w1, w2, fr, aa = heff2_problem(args.sites, args.bond, 6.0, seed=0, device=dev)
+    torch.cuda.synchronize()
+    log(f"built in {time.perf_counter() - t0:.1f}s: aa {aa.structure.size} elements, "
+        f"{aa.structure.num_blocks} blocks; FL {fl.structure.size}; FR {fr.structure.size}")
+
+    arms = [a for a in args.arms.split(",") if a]
+    ops_ = {p: Heff2(fl, w1, w2, fr, p) for p in set(arms) | {"fp64"}}
+    flops = ops_["fp64"].flops(aa.structure)
+    results = {}
+    outs = {}
+    for p in arms:
+        op = ops_[p]
+        op.apply_data(aa.structure, aa.data)  # compile
+        torch.cuda.synchronize()
+        dt = time_arm(lambda: op.apply_data(aa.structure, aa.data), args.steps, args.warmup)
+        outs[p] = op.apply_data(aa.structure, aa.data)
+        results[p] = {"matvecs_per_s": 1.0 / dt, "ms": dt * 1e3,
+                      "effective_tflops": flops / dt / 1e12}
+        log(f"arm {p}: {dt * 1e3:.3f} ms/matvec, {1 / dt:.1f} matvecs/s")
+
+    # accuracy of each arm's single matvec against the FP64 engine
+    ref = outs.get("fp64", ops_["fp64"].apply_data(aa.structure, aa.data))
+    for p in arms:
+        d = (outs[p] - ref).norm() / ref.norm()
+        results[p]["matvec_rel_err_vs_fp64"] = float(d)
+
+    if not args.no_blocks:
+        bfl, bw1, bw2, bfr, baa = (BlockTensor.from_symmetric(t) for t in (fl, w1, w2, fr, aa))
+        fn = lambda: blocks_heff2(bfl, bw1, bw2, bfr, baa)
+        dt = time_arm(fn, max(3, args.steps // 10), 1)
+        o = fn()
+        got = torch.cat([b.reshape(-1) for b in o.blocks])
+        results["fp64_blocks"] = {"matvecs_per_s": 1.0 / dt, "ms": dt * 1e3,
+                                  "matvec_rel_err_vs_fp64": float((got - ref).norm() / ref.norm())}
+        log(f"arm fp64_blocks: {dt * 1e3:.3f} ms/matvec")
+
+    # GEMM kernel timing over a timed pass of the headline arm
+    head = "fp32" if "fp32" in arms else arms[0]
+    ev: dict = {}
+    for _ in range(args.steps):
+        ops_[head].apply_data(aa.structure, aa.data, events=ev)
+    torch.cuda.synchronize()
+    per_step = {k: (sum(e0.elapsed_time(e1) for e0, e1, _ in v) / len(v), v[0][2]) for k, v in ev.items()}
+    dom = max(per_step, key=lambda k: per_step[k][1])
+    big = [k for k in per_step if per_step[k][1] >= 0.05 * flops]
+    gemm_ms = sum(per_step[k][0] for k in big)
+    gemm_flops = sum(per_step[k][1] for k in big)
+    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
+                "frac": achieved / PEAK_TFLOPS[head], "traffic": None,
+                "kernel": f"tn_gemm_grouped ({head}) H_eff steps {sorted(big)}",
+                "avg_launch_ms": {str(k): per_step[k][0] for k in per_step},
+                "flops_per_launch": {str(k): per_step[k][1] for k in per_step}}
+
+    # energy agreement: FP32/BF16x3 Krylov + FP64 Rayleigh quotient vs all-FP64
+    e64 = restarted_lanczos(ops_["fp64"], aa)
+    energy = {"E_fp64": e64}
+    for p in arms:
+        if p == "fp64":
+            continue
+        ep = restarted_lanczos(ops_[p], aa, refine=ops_["fp64"])
+        energy[f"E_{p}"] = ep
+        energy[f"dE_rel_{p}"] = abs(ep - e64) / abs(e64)
+    log(f"energies {energy}")
+
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU baseline (oracle, NumPy FP64)")
+        cpu, ref_cpu = cpu_baseline(fl, w1, w2, fr, aa)
+        got64 = ref.to("cpu").numpy()
+        results["fp64"]["rel_err_vs_oracle"] = float(np.linalg.norm(got64 - ref_cpu) / np.linalg.norm(ref_cpu))
+        log(f"cpu baseline {cpu['value']:.3f} matvecs/s")
+
+    v = results[head]["matvecs_per_s"]
+    line = {
+        "metric": "H_eff matvecs/s FP32-MFMA vs FP64 at |dE|<1e-8",
+        "value": v,
+        "unit": "matvecs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": results[head]["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if head != "fp64" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"two-site H_eff, U(1) Heisenberg L={args.sites}, D={args.bond} centre bond "
+                        f"(22 odd-charge sectors, Gaussian sigma=6), MPO bond 5, phys dim 2",
+            "aa_elements": aa.structure.size,
+            "flops_per_matvec": flops,
+        },
+        "arms": results,
+        "speedup_fp32_vs_fp64_blocks": (v / results["fp64_blocks"]["matvecs_per_s"]) if "fp64_blocks" in results else None,
+        "speedup_fp32_vs_cpu": (v / cpu["value"]) if cpu else None,
+        "energy": energy,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,61 @@
+"""ORACLE -- test infrastructure only.  Exact ground-state energies of open spin-1/2 XXZ
+chains, the independent pins for the DMRG path.
+
+* ``xxz_sz0_energy`` -- sparse exact diagonalization in the S^z_tot = 0 block, the
+  generalization to XXZ of the reference's ``heisenberg_sz0`` oracle
+  (``tests/integration/test_dmrg.py:120-139``), ``H = sum_i (delta Sz Sz + (S+S- + S-S+)/2)``.
+* ``xx_energy`` -- closed form at delta = 0 (free fermions by Jordan-Wigner):
+  ``E0 = sum_{k: eps_k < 0} eps_k``, ``eps_k = cos(pi k / (L+1))``.
+* Reference literals: ``E_OBC_12``, ``E_OBC_20`` (``test_dmrg.py:82-83``) and the MPSKit
+  ``N32``/``N64`` energies of ``tests/fixtures/mpskit_heisenberg.json``.
+"""
+
+from __future__ import annotations
+
+from itertools import combinations
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+E_OBC_12 = -5.142090632840532
+E_OBC_20 = -8.682473334398956
+E_MPSKIT_N32 = -13.997315618224897
+E_MPSKIT_N64 = -28.175424859745323
+
+
+def xxz_sz0_energy(n_sites: int, delta: float = 1.0) -> float:
+    if n_sites % 2:
+        raise ValueError("S^z_tot = 0 needs an even chain")
+    states = []
+    for ups in combinations(range(n_sites), n_sites // 2):
+        s = 0
+        for u in ups:
+            s |= 1 << u
+        states.append(s)
+    index = {s: i for i, s in enumerate(states)}
+    rows, cols, vals = [], [], []
+    for i, s in enumerate(states):
+        diag = 0.0
+        for b in range(n_sites - 1):
+            x = (s >> b) & 1
+            y = (s >> (b + 1)) & 1
+            diag += delta * (0.25 if x == y else -0.25)
+            if x != y:
+                t = s ^ (1 << b) ^ (1 << (b + 1))
+                rows.append(index[t])
+                cols.append(i)
+                vals.append(0.5)
+        rows.append(i)
+        cols.append(i)
+        vals.append(diag)
+    n = len(states)
+    h = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    if n < 200:
+        return float(np.linalg.eigvalsh(h.toarray())[0])
+    return float(spla.eigsh(h, k=1, which="SA", tol=1e-14)[0][0])
+
+
+def xx_energy(n_sites: int) -> float:
+    eps = np.cos(np.pi * np.arange(1, n_sites + 1) / (n_sites + 1))
+    return float(eps[eps < 0].sum())

@@ -15,8 +15,8 @@
 //   TN_PREC_BF16X3 v_mfma_f32_16x16x32_bf16, BK = 32; each FP32 operand is split into
 //                  hi = bf16(x), lo = bf16(x - hi) while staging to LDS and the product is
 //                  hi*hi + hi*lo + lo*hi with FP32 accumulation.
-// All three share the 16x16 accumulator layout: lane l, register j holds
-// C[(l/16)*4 + j][l%16].
+// Accumulator layout (16x16 tile): FP32 results -- lane l, register j holds
+// C[(l/16)*4 + j][l%16]; FP64 results -- C[4*j + l/16][l%16].
 #include "tn_common.hpp"
 
 namespace {
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(
         __syncthreads();
       }
     }
-    // ---- epilogue: lane l, register r -> row (l/16)*4 + r, col l%16 ----
+    // ---- epilogue ----
 #pragma unroll
     for (int i = 0; i < SM; ++i)
 #pragma unroll
@@ -216,7 +216,10 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(
         const int col = n0 + wn0 + j * 16 + (lane & 15);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = m0 + wm0 + i * 16 + (lane >> 4) * 4 + r;
+          // v_mfma_f64_16x16x4 spreads rows over registers (row 4r + l/16); the FP32
+          // accumulators keep four consecutive rows per lane group (row 4(l/16) + r)
+          const int rr = PREC == TN_PREC_FP64 ? r * 4 + (lane >> 4) : (lane >> 4) * 4 + r;
+          const int row = m0 + wm0 + i * 16 + rr;
           if (row < M && col < N) C[c_off + row * ldc + col] = acc[i][j][r];
         }
       }

@@ -257,6 +257,18 @@ def inplace_map(S: TensorStructure, D: TensorStructure, perm, terms):
     return offs if seen.all() else None
 
 
+def scatter_map(S: TensorStructure, D: TensorStructure, perm, terms):
+    """Offsets, inside D's buffer, of the target of each of S's blocks when the map is a
+    pure re-indexing (the output-side dual of ``inplace_map``); else None."""
+    if inplace_map(S, D, perm, terms) is None:
+        return None
+    offs = np.empty(S.num_blocks, dtype=np.int64)
+    dl = D.layout.offsets
+    for i, j, _ in terms:
+        offs[i] = dl[j]
+    return offs
+
+
 # ----------------------------------------------------------------------------- gemm
 
 _TILES = ((128, 128), (128, 64), (64, 128), (64, 64), (32, 32))
@@ -422,7 +434,7 @@ class ContractionExec:
             cur = t.new_structure
         self.out_structure = cur
         self.c_structure = Cp
-        c_offs = inplace_map(Cp, cur, plan[0], plan[1])
+        c_offs = scatter_map(Cp, cur, plan[0], plan[1])
         if c_offs is None:
             self.c_program = recouple_program(Cp, cur, tuple(plan[0]), tuple(plan[1]))
             c_offs = Cp.layout.offsets
@@ -462,7 +474,9 @@ class ContractionExec:
         _ = n_out_b
 
     def run(self, a_data: torch.Tensor, b_data: torch.Tensor, precision=None, tile=None,
-            out: torch.Tensor | None = None) -> torch.Tensor:
+            out: torch.Tensor | None = None, events: list | None = None) -> torch.Tensor:
+        """Run the contraction; when ``events`` is a list, the GEMM launch is bracketed by
+        two recorded events appended as ``(start, end, flops)``."""
         dtype, dev = a_data.dtype, a_data.device
         if b_data.dtype != dtype:
             raise TypeError("tensordot: operands have different dtypes")
@@ -478,12 +492,17 @@ class ContractionExec:
             bp = b_data
         if out is None:
             out = torch.empty(self.out_structure.size, dtype=dtype, device=dev)
-        if self.c_program is None:
-            self.gemm.run(ap, bp, out, precision, tile)
-        else:
-            cp = torch.empty(self.c_structure.size, dtype=dtype, device=dev)
-            self.gemm.run(ap, bp, cp, precision, tile)
-            self.c_program.run(cp, out)
+        target = out if self.c_program is None else torch.empty(
+            self.c_structure.size, dtype=dtype, device=dev)
+        if events is not None:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+        self.gemm.run(ap, bp, target, precision, tile)
+        if events is not None:
+            e1.record()
+            events.append((e0, e1, self.flops))
+        if self.c_program is not None:
+            self.c_program.run(target, out)
         return out
 
 

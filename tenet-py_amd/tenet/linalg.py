@@ -1,0 +1,339 @@
+"""Blockwise factorizations on the device: ``svd``, ``svd_truncated``, ``select_bond``, ``lq``.
+
+Lowering (reference ``ops/linalg.py:72-90``): repartition to ``(left | right)``, then one
+``tn_recouple`` launch gathers every block into one dense matrix per coupled sector, all
+packed in a single device buffer.  Each sector matrix is factorized in FP64 on the GPU
+(rocSOLVER through torch-ROCm), the truncation decision runs on the host over the O(D)
+singular values with the reference's exact keep rule (global spectrum sorted by
+``(-sigma, sector, index)``, cutoff modes, greedy walk under the dense budget
+``sum qdim(c)``; reference ``linalg.py:1079-1324``), and one more ``tn_recouple`` launch
+per factor scatters the kept columns/rows back into block storage.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from functools import lru_cache
+
+import numpy as np
+import torch
+
+from tenet.engine import RecoupleProgram, require_device
+from tenet.ops import repartition
+from tenet.plans import map_layout
+from tenet.space import IN, OUT, GradedSpace, Leg
+from tenet.structure import TensorStructure
+from tenet.symmetry import StructureChangingError
+from tenet.tensor import SymmetricTensor
+
+__all__ = ["svd", "svd_truncated", "select_bond", "BondSelection", "lq", "to_matrices", "from_matrices"]
+
+_CUTOFF_MODES = ("abs", "rel", "sum2", "rsum2", "sum1", "rsum1")
+
+
+# ----------------------------------------------------------------------------- matrix lowering
+
+
+class _MatrixLayout:
+    """Packed sector matrices of a structure and the two recouple programs to/from them."""
+
+    def __init__(self, s: TensorStructure) -> None:
+        lay = map_layout(s)
+        self.sectors = lay.sectors
+        self.shapes = {c: lay.shape(c) for c in lay.sectors}
+        self.offsets = {}
+        acc = 0
+        for c in lay.sectors:
+            self.offsets[c] = acc
+            r, k = self.shapes[c]
+            acc += r * k
+        self.size = acc
+        bl = s.layout
+        n_out = len(s.out_axes)
+        order = (*s.out_axes, *s.in_axes)
+        to_regions, from_regions = [], []
+        for c, cells in lay.grid:
+            rb, cb = lay.row_bands(c), lay.col_bands(c)
+            ld = self.shapes[c][1]
+            base = self.offsets[c]
+            for (_, ro, dr), row in zip(rb, cells):
+                for (_, co, dc), i in zip(cb, row):
+                    shape = tuple(int(bl.shapes[i][a]) for a in order)
+                    blk = tuple(int(bl.strides[i][a]) for a in order)
+                    mat = [0] * len(order)
+                    st = ld
+                    for d in range(n_out - 1, -1, -1):
+                        mat[d] = st
+                        st *= shape[d]
+                    st = 1
+                    for d in range(len(order) - 1, n_out - 1, -1):
+                        mat[d] = st
+                        st *= shape[d]
+                    moff = base + ro * ld + co
+                    to_regions.append((moff, shape, tuple(mat), blk, [(int(bl.offsets[i]), 1.0)]))
+                    from_regions.append((int(bl.offsets[i]), shape, blk, tuple(mat), [(moff, 1.0)]))
+        self.to_prog = RecoupleProgram(to_regions, s.size, self.size)
+        self.from_prog = RecoupleProgram(from_regions, self.size, s.size)
+
+
+@lru_cache(maxsize=1024)
+def _matrix_layout(s: TensorStructure) -> _MatrixLayout:
+    return _MatrixLayout(s)
+
+
+def to_matrices(t: SymmetricTensor) -> dict:
+    """``{c: B_c}`` device matrices (views into one packed buffer), rows OUT, columns IN
+    (reference ``map_view.py:238-279``)."""
+    require_device(t)
+    ml = _matrix_layout(t.structure)
+    buf = torch.empty(ml.size, dtype=t.dtype, device=t.device)
+    ml.to_prog.run(t.data, buf)
+    return {
+        c: buf[ml.offsets[c] : ml.offsets[c] + ml.shapes[c][0] * ml.shapes[c][1]].view(ml.shapes[c])
+        for c in ml.sectors
+    }
+
+
+def from_matrices(structure: TensorStructure, mats: dict) -> SymmetricTensor:
+    """Inverse of ``to_matrices`` (reference ``map_view.py:282-335``)."""
+    ml = _matrix_layout(structure)
+    if set(mats) != set(ml.sectors):
+        raise ValueError("from_matrices: sector set does not match the structure")
+    ref = next(iter(mats.values()))
+    buf = torch.empty(ml.size, dtype=ref.dtype, device=ref.device)
+    for c in ml.sectors:
+        m = mats[c]
+        if tuple(m.shape) != tuple(ml.shapes[c]):
+            raise ValueError(f"from_matrices: sector {c!r} has shape {tuple(m.shape)}, expected {ml.shapes[c]}")
+        buf[ml.offsets[c] : ml.offsets[c] + m.numel()].view(ml.shapes[c]).copy_(m)
+    out = torch.empty(structure.size, dtype=ref.dtype, device=ref.device)
+    ml.from_prog.run(buf, out)
+    return SymmetricTensor(structure, out)
+
+
+def _lower(t: SymmetricTensor, axes):
+    if axes is None:
+        axes = (t.structure.out_axes, t.structure.in_axes)
+    left, right = axes
+    m = repartition(t, left, right)
+    return m, to_matrices(m)
+
+
+# ----------------------------------------------------------------------------- keep rule
+
+
+@dataclass(frozen=True)
+class BondSelection:
+    """The truncation decision (reference ``linalg.py:1159-1271``)."""
+
+    bond: GradedSpace
+    dense_dim: float
+    reduced_dim: int
+    kept: tuple
+    discarded: tuple
+    discarded_weight: float
+    next_dense_cost: float
+    max_bond: int | None
+    scale: float
+
+    @property
+    def next_multiplet(self):
+        return self.discarded[0] if self.discarded else None
+
+    @property
+    def undershoot(self):
+        return None if self.max_bond is None else self.max_bond - self.dense_dim
+
+
+def _validate(max_bond, cutoff, cutoff_mode, renorm, caller):
+    if max_bond is None and cutoff is None:
+        raise ValueError(f"{caller} needs at least one of max_bond or cutoff")
+    if max_bond is not None and max_bond <= 0:
+        raise ValueError(f"max_bond must be a positive dense bond dimension, got {max_bond!r}")
+    if cutoff is not None and cutoff < 0:
+        raise ValueError(f"cutoff must be non-negative, got {cutoff!r}")
+    if not isinstance(cutoff_mode, str) or cutoff_mode not in _CUTOFF_MODES:
+        raise ValueError(f"unknown cutoff_mode {cutoff_mode!r}; expected one of {_CUTOFF_MODES}")
+    if not isinstance(renorm, bool):
+        raise TypeError(f"renorm must be a bool, got {renorm!r}")
+
+
+def _spectrum(values: dict) -> list:
+    """``[(sigma, c, i)]`` sorted by ``(-sigma, c, i)``; ``values`` are host arrays."""
+    entries = [(float(s), c, i) for c, arr in values.items() for i, s in enumerate(arr)]
+    entries.sort(key=lambda e: (-e[0], e[1], e[2]))
+    return entries
+
+
+def _admissible(spectrum, qdim, cutoff, mode) -> int:
+    if cutoff is None:
+        return len(spectrum)
+    if mode == "abs":
+        return sum(1 for s, _, _ in spectrum if s > cutoff)
+    if mode == "rel":
+        thr = cutoff * spectrum[0][0]
+        return sum(1 for s, _, _ in spectrum if s > thr)
+    power = 2 if mode in ("sum2", "rsum2") else 1
+    weights = [qdim(c) * s**power for s, c, _ in spectrum]
+    thr = cutoff * sum(weights) if mode.startswith("r") else cutoff
+    kept, dropped = len(spectrum), 0.0
+    for w in reversed(weights):
+        if dropped + w >= thr:
+            break
+        dropped += w
+        kept -= 1
+    return kept
+
+
+def _decide(spectrum, provider, max_bond, cutoff, cutoff_mode, renorm, caller) -> BondSelection:
+    qdim = provider.qdim
+    if not spectrum:
+        raise ValueError(f"{caller}: this tensor has no singular values at all")
+    admissible = _admissible(spectrum, qdim, cutoff, cutoff_mode)
+    keep: dict = {}
+    kept = []
+    kept_w, dense = 0.0, 0.0
+    for entry in spectrum[:admissible]:
+        s, c, _ = entry
+        if max_bond is not None and dense + qdim(c) > max_bond:
+            break
+        dense += qdim(c)
+        keep[c] = keep.get(c, 0) + 1
+        kept.append(entry)
+        kept_w += qdim(c) * s * s
+    if not keep:
+        raise ValueError(
+            f"{caller}: cutoff={cutoff!r} in cutoff_mode={cutoff_mode!r} with max_bond={max_bond!r} "
+            "keeps no singular value at all"
+        )
+    discarded = tuple(spectrum[len(kept) :])
+    total = sum(qdim(c) * s * s for s, c, _ in spectrum)
+    return BondSelection(
+        bond=GradedSpace.new(provider, keep),
+        dense_dim=dense,
+        reduced_dim=len(kept),
+        kept=tuple(kept),
+        discarded=discarded,
+        discarded_weight=sum(qdim(c) * s * s for s, c, _ in discarded),
+        next_dense_cost=qdim(discarded[0][1]) if discarded else 0.0,
+        max_bond=max_bond,
+        scale=(total / kept_w) ** 0.5 if renorm else 1.0,
+    )
+
+
+# ----------------------------------------------------------------------------- factorizations
+
+
+def _svd_parts(mats: dict) -> dict:
+    """FP64 SVD of every sector matrix on the device; equal shapes are batched."""
+    parts = {}
+    by_shape: dict = {}
+    for c, m in mats.items():
+        by_shape.setdefault(tuple(m.shape), []).append(c)
+    for shape, cs in by_shape.items():
+        if len(cs) == 1:
+            u, s, vh = torch.linalg.svd(mats[cs[0]], full_matrices=False)
+            parts[cs[0]] = (u, s, vh)
+        else:
+            u, s, vh = torch.linalg.svd(torch.stack([mats[c] for c in cs]), full_matrices=False)
+            for k, c in enumerate(cs):
+                parts[c] = (u[k], s[k], vh[k])
+    return parts
+
+
+def _factors(m: SymmetricTensor, parts: dict, keep: dict, scale: float = 1.0):
+    provider = m.provider
+    bond = GradedSpace.new(provider, keep)
+    us = TensorStructure((*m.codomain, Leg(bond, IN)))
+    ss = TensorStructure((Leg(bond, OUT), Leg(bond, IN)))
+    vs = TensorStructure((Leg(bond, OUT), *m.domain))
+    u = from_matrices(us, {c: parts[c][0][:, :k] for c, k in keep.items()})
+    s = from_matrices(ss, {c: torch.diag(parts[c][1][:k] * scale) for c, k in keep.items()})
+    vh = from_matrices(vs, {c: parts[c][2][:k, :] for c, k in keep.items()})
+    return u, s, vh
+
+
+def svd(t: SymmetricTensor, axes=None, *, bond: GradedSpace | None = None):
+    """Compact SVD ``T = U S Vh`` per coupled sector, or its projection onto a
+    pre-decided ``bond`` (reference ``linalg.py:122-245``)."""
+    m, mats = _lower(t, axes)
+    parts = _svd_parts(mats)
+    full = {c: min(mats[c].shape) for c in mats}
+    if bond is None:
+        keep = full
+    else:
+        keep = {}
+        for c, k in bond.sectors:
+            if k > full.get(c, 0):
+                raise ValueError(
+                    f"svd: bond asks for {k} singular values in sector {c!r}, but the untruncated "
+                    f"bond degeneracy there is {full.get(c, 0)}"
+                )
+            keep[c] = k
+    return _factors(m, parts, keep)
+
+
+def select_bond(t: SymmetricTensor, axes=None, *, max_bond=None, cutoff=None, cutoff_mode="rsum2",
+                renorm=False) -> BondSelection:
+    """The truncation decision alone (reference ``linalg.py:1327-1438``)."""
+    _validate(max_bond, cutoff, cutoff_mode, renorm, "select_bond")
+    m, mats = _lower(t, axes)
+    parts = _svd_parts(mats)
+    return _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, "select_bond")
+
+
+def _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, caller):
+    cs = list(parts)
+    if not cs:
+        raise ValueError(f"{caller}: this tensor has no singular values at all")
+    flat = torch.cat([parts[c][1] for c in cs]).to("cpu").numpy()
+    values, o = {}, 0
+    for c in cs:
+        n = parts[c][1].numel()
+        values[c] = flat[o : o + n]
+        o += n
+    return _decide(_spectrum(values), m.provider, max_bond, cutoff, cutoff_mode, renorm, caller)
+
+
+def svd_truncated(t: SymmetricTensor, axes=None, *, max_bond=None, cutoff=None,
+                  cutoff_mode: str = "rsum2", renorm: bool = False):
+    """Truncating blockwise SVD with a global keep rule (reference ``linalg.py:1441-1612``).
+
+    Returns ``(U, S, Vh)``: ``U`` on ``(*left legs, bond IN)``, ``S`` diagonal on
+    ``(bond OUT, bond IN)``, ``Vh`` on ``(bond OUT, *right legs)``.
+    """
+    _validate(max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
+    m, mats = _lower(t, axes)
+    parts = _svd_parts(mats)
+    sel = _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
+    return _factors(m, parts, dict(sel.bond.sectors), sel.scale)
+
+
+def svd_truncated_with_selection(t, axes=None, *, max_bond=None, cutoff=None, cutoff_mode="rsum2",
+                                 renorm=False):
+    """``svd_truncated`` that also returns its ``BondSelection``."""
+    _validate(max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
+    m, mats = _lower(t, axes)
+    parts = _svd_parts(mats)
+    sel = _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
+    return (*_factors(m, parts, dict(sel.bond.sectors), sel.scale), sel)
+
+
+def lq(t: SymmetricTensor, axes=None):
+    """``T = L Q`` per coupled sector with ``Q`` row-orthonormal, bond ``min(rows, cols)``
+    (reference ``linalg.py:549-620``: QR of each sector matrix's transpose)."""
+    m, mats = _lower(t, axes)
+    provider = m.provider
+    keep = {c: min(b.shape) for c, b in mats.items()}
+    bond = GradedSpace.new(provider, keep)
+    ls, qs = {}, {}
+    for c, b in mats.items():
+        q, r = torch.linalg.qr(b.transpose(0, 1))
+        ls[c] = r.transpose(0, 1)
+        qs[c] = q.transpose(0, 1)
+    L = from_matrices(TensorStructure((*m.codomain, Leg(bond, IN))), ls)
+    Q = from_matrices(TensorStructure((Leg(bond, OUT), *m.domain)), qs)
+    return L, Q
+
+
+_ = StructureChangingError

@@ -1,0 +1,9 @@
+"""Tensor-network layer: MPS/MPO, environments, the two-site H_eff and DMRG."""
+
+from tenet.network.dmrg import DMRG_out, Sweep, dmrg_, lanczos, spectrum, sweep_
+from tenet.network.env import Env, ones
+from tenet.network.heff import Heff2
+from tenet.network.mps import MPO, MPS, from_dense
+
+__all__ = ["DMRG_out", "Env", "Heff2", "MPO", "MPS", "Sweep", "dmrg_", "from_dense", "lanczos",
+           "ones", "spectrum", "sweep_"]

@@ -1,0 +1,295 @@
+"""The two-site H_eff matvec compiled into a fixed device program.
+
+``Heff2(fl, w_n, w_{n+1}, fr, precision)`` plans the four contractions of the site-tensor
+path (reference ``network/env.py:416-432``, with the ``_composed`` bends of
+``env.py:60-90``) once per input structure and keeps, per step, either a compiled
+recouple program or nothing (U(1) bends are pure block re-indexings and move no data),
+plus one compiled block-sparse GEMM program.  Static operands (environments and MPO
+tensors) are bent and, for the FP32/BF16x3 arms, cast to FP32 once at construction; the
+Lanczos vector stays FP64 and is cast on entry/exit.  Every launch of one apply is a
+fixed sequence on the current stream, so the whole matvec can be captured as a hipGraph
+(``Heff2.graph``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from tenet import ops
+from tenet.engine import contraction_exec, inplace_map, recouple_program, require_device
+from tenet.plans import repartition_plan
+from tenet.space import OUT
+from tenet.structure import TensorStructure
+from tenet.tensor import SymmetricTensor
+
+__all__ = ["Heff2", "HEFF2_STEPS"]
+
+# (equation, bend) of the four contractions: right env, W_{n+1}, W_n, left env
+HEFF2_STEPS = (
+    ("apqr,rys->apqys", ""),
+    ("apqys,mQqy->apQms", "q"),
+    ("apQms,xPpm->aPQxs", "p"),
+    ("aPQxs,axB->BPQs", "a"),
+)
+
+
+def _bent(structure: TensorStructure, term: str, bend: str):
+    flip = set(bend)
+    outs = tuple(i for i, c in enumerate(term) if (structure.legs[i].side is OUT) != (c in flip))
+    ins = tuple(i for i in range(len(term)) if i not in outs)
+    return outs, ins, "".join(term[i] for i in (*outs, *ins))
+
+
+def _einsum_axes(t0: str, t1: str, out: str):
+    shared = [c for c in t0 if c in t1]
+    axes = (tuple(t0.index(c) for c in shared), tuple(t1.index(c) for c in shared))
+    free = [c for c in t0 if c not in shared] + [c for c in t1 if c not in shared]
+    return axes, tuple(free.index(c) for c in out)
+
+
+class _Chain:
+    """The compiled launch sequence for one input structure."""
+
+    def __init__(self, s_in: TensorStructure, statics) -> None:
+        self.steps = []
+        self.flops = 0
+        s = s_in
+        for (eq, bend), stat in zip(HEFF2_STEPS, statics):
+            lhs, out = eq.split("->")
+            ta, tb = lhs.split(",")
+            rep = None
+            if bend:
+                outs, ins, ta = _bent(s, ta, bend)
+                plan = repartition_plan(s, outs, ins)
+                offs = inplace_map(s, plan.new_structure, plan.perm, plan.terms)
+                if offs is None or not (offs == plan.new_structure.layout.offsets).all():
+                    rep = recouple_program(s, plan.new_structure, tuple(plan.perm), tuple(plan.terms))
+                s = plan.new_structure
+            axes, out_axes = _einsum_axes(ta, stat[1], out)
+            ex = contraction_exec(s, stat[0].structure, axes, out_axes)
+            self.flops += ex.flops
+            self.steps.append((rep, s, ex, stat[0]))
+            s = ex.out_structure
+        if s != s_in:
+            raise ValueError("heff2: the output structure differs from the input's")
+        self.s_out = s
+
+    def run(self, x: torch.Tensor, precision: str, events: list | None = None) -> torch.Tensor:
+        cur = x
+        for k, (rep, s, ex, stat) in enumerate(self.steps):
+            if rep is not None:
+                nxt = torch.empty(s.size, dtype=cur.dtype, device=cur.device)
+                rep.run(cur, nxt)
+                cur = nxt
+            ev = None if events is None else events.setdefault(k, [])
+            cur = ex.run(cur, stat.data, precision, events=ev)
+        return cur
+
+
+def _dense(t: SymmetricTensor):
+    """Host dense array of a small abelian tensor (axes over space sectors, sorted)."""
+    offs = []
+    for leg in t.legs:
+        o, acc = {}, 0
+        for sec, m in leg.space.sectors:
+            o[sec] = acc
+            acc += m
+        offs.append((o, acc))
+    out = np.zeros(tuple(n for _, n in offs))
+    for key, blk in zip(t.structure.block_order, t.numpy_blocks()):
+        secs = t.structure.axis_sectors(key)
+        sl = tuple(slice(o[x], o[x] + leg.degeneracy(x)) for (o, _), leg, x in zip(offs, t.legs, secs))
+        out[sl] = blk
+    return out, [o for o, _ in offs]
+
+
+class _FusedChain:
+    """Abelian fast path: three launches, no permutation pass.
+
+    1. ``t1[a,p,q,y,s] = sum_r aa[a,p,q,r] FR[r,y,s]`` -- block GEMM reading ``aa`` and
+       ``FR`` in place and writing ``t1`` blocks in place.
+    2. ``t3[a,x,P,Q,s] = sum_{p,q,y} W12[x,P,Q; p,q,y] t1[a,p,q,y,s]`` with the two-site MPO
+       ``W12 = sum_m W_n[x,P,p,m] W_{n+1}[m,Q,q,y]`` merged on the host -- one
+       ``tn_mpo_apply`` pass per (a-sector, s-sector) slab pair, written straight into the
+       layout step 3 consumes.
+    3. ``out[B,P,Q,s] = sum_{a,x} FL[a,x,B] t3[a,x,P,Q,s]`` -- block GEMM with ``FL``
+       transposed in place, output blocks written where ``aa``'s structure puts them.
+
+    For an abelian symmetry every bend and braid coefficient is 1, so the site-tensor
+    path's four contractions are exactly these three steps on the dense-index level.
+    """
+
+    def __init__(self, s_in: TensorStructure, fr, fl, w_left, w_right, fr_term) -> None:
+        from tenet.engine import GemmProgram, _DeviceCache
+
+        axes, out_axes = _einsum_axes("apqr", fr_term, "apqys")
+        self.ex1 = contraction_exec(s_in, fr.structure, axes, out_axes)
+        s1 = self.ex1.out_structure
+        self.s_out = s_in
+        w1, o1 = _dense(w_left)   # (x, P, p, m)
+        w2, o2 = _dense(w_right)  # (m, Q, q, y)
+        w12 = np.einsum("xPpm,mQqy->xPQpqy", w1, w2)
+        # t1 channels per (a, s) sector pair
+        l1 = s1.layout
+        legs1 = s1.legs
+        by_as_in: dict = {}
+        for i, key in enumerate(s1.block_order):
+            a, p, q, y, sg = s1.axis_sectors(key)
+            sh = l1.shapes[i]
+            da, dp, dq, my, ds = (int(v) for v in sh)
+            for ip in range(dp):
+                for iq in range(dq):
+                    for iy in range(my):
+                        off = int(l1.offsets[i]) + ((ip * dq + iq) * my + iy) * ds
+                        dense = (o1[2][p] + ip, o2[2][q] + iq, o2[3][y] + iy)
+                        by_as_in.setdefault((a, sg), []).append((off, dp * dq * my * ds, dense, da, ds))
+        # t3 blocks: every (FL block (a,x,B)) x (output block (B,P,Q,s)) pair step 3 reads
+        lfl = fl.structure
+        out_by_b: dict = {}
+        for j, key in enumerate(s_in.block_order):
+            B, P, Q, sg = s_in.axis_sectors(key)
+            out_by_b.setdefault(B, []).append((j, P, Q, sg))
+        t3_blocks = []  # (a, x, P, Q, s, shape, fl_index, out_index)
+        for i, key in enumerate(lfl.block_order):
+            a, x, B = lfl.axis_sectors(key)
+            da, mx, db = (int(v) for v in lfl.layout.shapes[i])
+            for j, P, Q, sg in out_by_b.get(B, ()):
+                _, dP, dQ, ds = (int(v) for v in s_in.layout.shapes[j])
+                t3_blocks.append((a, x, P, Q, sg, (da, mx, dP, dQ, ds), i, j))
+        t3_off = np.zeros(len(t3_blocks), dtype=np.int64)
+        acc = 0
+        for k, b in enumerate(t3_blocks):
+            t3_off[k] = acc
+            acc += int(np.prod(b[5]))
+        self.t3_size = acc
+        # slab-pair jobs of the MPO pass
+        out_by_as: dict = {}
+        for k, (a, x, P, Q, sg, (da, mx, dP, dQ, ds), _, _) in enumerate(t3_blocks):
+            for ix in range(mx):
+                for iP in range(dP):
+                    for iQ in range(dQ):
+                        off = int(t3_off[k]) + ((ix * dP + iP) * dQ + iQ) * ds
+                        dense = (o1[0][x] + ix, o1[1][P] + iP, o2[1][Q] + iQ)
+                        out_by_as.setdefault((a, sg), []).append((off, mx * dP * dQ * ds, dense, da, ds))
+        jobs, ins, outs, coef, tiles = [], [], [], [], []
+        for (a, sg), olist in out_by_as.items():
+            ilist = by_as_in.get((a, sg), [])
+            if len(ilist) > 32:
+                raise ValueError("two-site MPO slab pair has more than 32 input channels")
+            R, S = olist[0][3], olist[0][4]
+            cm = np.array([[w12[o[2] + i[2]] for i in ilist] for o in olist], dtype=np.float64).reshape(len(olist), len(ilist))
+            j = len(jobs)
+            jobs.append((R, S, len(ins), len(ins) + len(ilist), len(outs), len(outs) + len(olist), len(coef), 0))
+            ins.extend((i[0], i[1]) for i in ilist)
+            outs.extend((o[0], o[1]) for o in olist)
+            coef.extend(cm.reshape(-1).tolist())
+            nt = -(-(R * S) // 256)
+            tiles.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
+            for off, rs, _, _, _ in ilist:
+                if off + (R - 1) * rs + S > s1.size:
+                    raise ValueError("MPO pass reads outside t1")
+            for off, rs, _, _, _ in olist:
+                if off + (R - 1) * rs + S > self.t3_size:
+                    raise ValueError("MPO pass writes outside t3")
+        self.mpo_njobs = len(jobs)
+        tiles = np.concatenate(tiles) if tiles else np.zeros((0, 2), np.int64)
+        self.mpo_ntiles = len(tiles)
+        self.mpo_tables = _DeviceCache(
+            jobs=np.asarray(jobs, dtype=np.int64).reshape(-1),
+            ins=np.asarray(ins, dtype=np.int64).reshape(-1),
+            outs=np.asarray(outs, dtype=np.int64).reshape(-1),
+            coef=np.asarray(coef, dtype=np.float64),
+            tiles=tiles.reshape(-1),
+        )
+        self.mpo_bytes_elems = s1.size + self.t3_size
+        # step 3 problems: output block j = sum over its (a, x) t3 blocks
+        segs_by_out: dict = {}
+        for k, (a, x, P, Q, sg, (da, mx, dP, dQ, ds), i, j) in enumerate(t3_blocks):
+            db = int(lfl.layout.shapes[i][2])
+            segs_by_out.setdefault(j, []).append(
+                (int(lfl.layout.offsets[i]), int(t3_off[k]), da * mx, db, dP * dQ * ds))
+        problems = []
+        ol = s_in.layout
+        for j in range(s_in.num_blocks):
+            db, dP, dQ, ds = (int(v) for v in ol.shapes[j])
+            problems.append((db, dP * dQ * ds, int(ol.offsets[j]), dP * dQ * ds, segs_by_out.get(j, [])))
+        self.gemm4 = GemmProgram(problems, lfl.size, self.t3_size, s_in.size, trans_a=True)
+        self.flops = self.ex1.flops + self.gemm4.flops
+        self.fl, self.fr = fl, fr
+
+    def run(self, x: torch.Tensor, precision: str, events: dict | None = None) -> torch.Tensor:
+        from tenet import _native
+
+        ev1 = None if events is None else events.setdefault(0, [])
+        t1 = self.ex1.run(x, self.fr.data, precision, events=ev1)
+        t3 = torch.empty(self.t3_size, dtype=x.dtype, device=x.device)
+        tb = self.mpo_tables.get(x.device)
+        _native.check(_native.lib().tn_mpo_apply(
+            _native.TN_F64 if x.dtype == torch.float64 else _native.TN_F32,
+            t1.data_ptr(), t3.data_ptr(), tb["jobs"].data_ptr(), self.mpo_njobs,
+            tb["ins"].data_ptr(), tb["outs"].data_ptr(), tb["coef"].data_ptr(),
+            tb["tiles"].data_ptr(), self.mpo_ntiles, _native.stream_ptr()))
+        out = torch.empty(self.s_out.size, dtype=x.dtype, device=x.device)
+        if events is not None:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+        self.gemm4.run(self.fl.data, t3, out, precision)
+        if events is not None:
+            e1.record()
+            events.setdefault(3, []).append((e0, e1, self.gemm4.flops))
+        return out
+
+
+class Heff2:
+    """Compiled ``H_eff`` of one bond at a fixed arithmetic (``fp64``, ``fp32``, ``bf16x3``)."""
+
+    def __init__(self, fl: SymmetricTensor, w_left: SymmetricTensor, w_right: SymmetricTensor,
+                 fr: SymmetricTensor, precision: str = "fp64", fused: bool | None = None) -> None:
+        if precision not in ("fp64", "fp32", "bf16x3"):
+            raise ValueError(f"unknown precision {precision!r}")
+        require_device(fl, w_left, w_right, fr)
+        self.precision = precision
+        self.fl, self.fr = fl, fr
+        dt = torch.float64 if precision == "fp64" else torch.float32
+        self.dtype = dt
+        statics = []
+        for (eq, bend), t in zip(HEFF2_STEPS, (fr, w_right, w_left, fl)):
+            tb = eq.split("->")[0].split(",")[1]
+            if bend:
+                outs, ins, tb = _bent(t.structure, tb, bend)
+                t = ops.repartition(t, outs, ins)
+            statics.append((ops.astype(t, dt), tb))
+        self._statics = statics
+        self._chains: dict = {}
+        self._graphs: dict = {}
+        self.fused = bool(getattr(fl.provider, "unique_fusion", False)) if fused is None else fused
+        if self.fused:
+            self._fl_w = ops.astype(fl, dt)
+            self._w = (w_left, w_right)
+
+    def chain(self, s: TensorStructure):
+        c = self._chains.get(s)
+        if c is None:
+            if self.fused:
+                c = _FusedChain(s, self._statics[0][0], self._fl_w, self._w[0], self._w[1],
+                                self._statics[0][1])
+            else:
+                c = _Chain(s, self._statics)
+            self._chains[s] = c
+        return c
+
+    def flops(self, s: TensorStructure) -> int:
+        return self.chain(s).flops
+
+    def apply_data(self, structure: TensorStructure, x: torch.Tensor, events: dict | None = None) -> torch.Tensor:
+        """``H_eff x`` on a flat buffer of ``structure``; returns a buffer of x's dtype.
+        ``events`` (a dict) collects per-step GEMM events for kernel timing."""
+        c = self.chain(structure)
+        if x.dtype != self.dtype:
+            y = c.run(ops.astype(SymmetricTensor(structure, x), self.dtype).data, self.precision, events)
+            return ops.astype(SymmetricTensor(c.s_out, y), x.dtype).data
+        return c.run(x, self.precision, events)
+
+    def __call__(self, aa: SymmetricTensor) -> SymmetricTensor:
+        return SymmetricTensor(aa.structure, self.apply_data(aa.structure, aa.data))

@@ -1,0 +1,146 @@
+"""MPS and MPO containers on device-resident tensors.
+
+``MPS`` keeps every site on the partition ``(left bond OUT, phys OUT | right bond IN)``
+through a write barrier (reference ``network/mps.py:55-352,677-697``).  ``MPO.from_w``
+builds the site-tensor operator ``(wl IN, p OUT, p IN, wr OUT)`` from one dense ``W``
+(reference ``mps.py:2780-2845``); it carries no edge description, so ``Env.heff2`` takes
+the site-tensor path.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Iterable, Sequence
+
+import numpy as np
+import torch
+
+from tenet import linalg, ops
+from tenet.space import IN, OUT, GradedSpace, Leg
+from tenet.structure import TensorStructure
+from tenet.tensor import SymmetricTensor, default_device
+
+__all__ = ["MPS", "MPO", "from_dense"]
+
+
+def _as_site(t: SymmetricTensor) -> SymmetricTensor:
+    if t.ndim != 3:
+        raise ValueError(f"an MPS site tensor is rank 3, got rank {t.ndim}")
+    return ops.repartition(t, (0, 1), (2,))
+
+
+class MPS:
+    """A finite MPS: ``sites[n]`` on ``(bond_n OUT, phys OUT | bond_{n+1} IN)``."""
+
+    def __init__(self, sites: Iterable[SymmetricTensor], center: int | None = None) -> None:
+        self.sites = [_as_site(t) for t in sites]
+        self.center = center
+
+    def __len__(self) -> int:
+        return len(self.sites)
+
+    def __getitem__(self, n: int) -> SymmetricTensor:
+        return self.sites[n]
+
+    def __setitem__(self, n: int, t: SymmetricTensor) -> None:
+        if not isinstance(n, int):
+            raise TypeError(f"MPS index must be an int, got {type(n).__name__}")
+        self.sites[n] = _as_site(t)
+
+    def __iter__(self):
+        return iter(self.sites)
+
+    def copy(self) -> "MPS":
+        return MPS(self.sites, self.center)
+
+    @classmethod
+    def random(cls, phys: GradedSpace, bonds: Sequence[GradedSpace], *, seed: int = 0,
+               dtype=None, device=None) -> "MPS":
+        """Site ``i`` drawn with seed ``seed + i`` (reference ``mps.py:162-190``)."""
+        return cls(
+            SymmetricTensor.random(
+                (Leg(bonds[i], OUT), Leg(phys, OUT), Leg(bonds[i + 1], IN)),
+                seed=seed + i, dtype=dtype, device=device,
+            )
+            for i in range(len(bonds) - 1)
+        )
+
+    def canonize_(self, to: int = 0) -> "MPS":
+        """Right-canonical form by an LQ sweep from the right (reference ``mps.py:312-352``)."""
+        if to != 0:
+            raise NotImplementedError("only to=0 (right-canonical) is implemented")
+        for n in range(len(self) - 1, 0, -1):
+            left, q = linalg.lq(self[n], ((0,), (1, 2)))
+            self[n] = q
+            self[n - 1] = ops.einsum("apx,xy->apy", self[n - 1], left)
+        self[0] = ops.divide(self[0], ops.norm(self[0]))
+        self.center = to
+        return self
+
+
+def from_dense(array, legs: Sequence[Leg], *, atol: float = 1e-12, device=None) -> SymmetricTensor:
+    """Blocks of a dense array whose every axis runs over its leg's sectors in canonical
+    order (abelian providers: no Clebsch-Gordan factor).  Raises when the array has weight
+    outside the symmetric block set."""
+    s = TensorStructure(tuple(legs))
+    a = np.asarray(array, dtype=np.float64)
+    offs = []
+    for leg in s.legs:
+        if not getattr(leg.provider, "unique_fusion", False):
+            raise NotImplementedError("from_dense is implemented for abelian providers")
+        o, acc = {}, 0
+        for sec, m in leg.space.sectors:
+            o[sec] = acc
+            acc += m
+        offs.append(o)
+        if acc != a.shape[len(offs) - 1]:
+            raise ValueError(f"axis {len(offs) - 1} has dimension {a.shape[len(offs) - 1]}, leg needs {acc}")
+    flat = np.empty(s.size)
+    check = np.zeros_like(a)
+    lay = s.layout
+    for i, key in enumerate(s.block_order):
+        secs = s.axis_sectors(key)
+        sl = tuple(slice(o[x], o[x] + leg.degeneracy(x)) for o, leg, x in zip(offs, s.legs, secs))
+        blk = a[sl]
+        flat[int(lay.offsets[i]) : int(lay.offsets[i] + lay.sizes[i])] = blk.reshape(-1)
+        check[sl] = blk
+    resid = float(np.abs(a - check).max()) if a.size else 0.0
+    if resid > atol * max(1.0, float(np.abs(a).max())):
+        raise ValueError(f"from_dense: the array is not symmetric on these legs (residual {resid:.3g})")
+    dev = torch.device(device) if device is not None else default_device()
+    return SymmetricTensor(s, torch.from_numpy(flat).to(dev))
+
+
+class MPO:
+    """A finite MPO of site tensors ``(wl IN, p OUT, p IN, wr OUT)``."""
+
+    def __init__(self, sites: Iterable[SymmetricTensor]) -> None:
+        self.sites = list(sites)
+        self.edges = None
+
+    def __len__(self) -> int:
+        return len(self.sites)
+
+    def __getitem__(self, n: int) -> SymmetricTensor:
+        return self.sites[n]
+
+    def __iter__(self):
+        return iter(self.sites)
+
+    def materialize(self) -> "MPO":
+        return self
+
+    @classmethod
+    def from_w(cls, w, n_sites: int, *, phys: GradedSpace, bond: GradedSpace,
+               boundary: GradedSpace, start: int, end: int, device=None) -> "MPO":
+        """One bulk ``W[wl, p_out, p_in, wr]`` with boundary rows/columns cut at ``start``
+        and ``end`` (reference ``mps.py:2780-2845``)."""
+        w = np.asarray(w, dtype=np.float64)
+
+        def make(arr, left, right):
+            legs = (Leg(left, IN), Leg(phys, OUT), Leg(phys, IN), Leg(right, OUT))
+            return from_dense(arr, legs, device=device)
+
+        first = make(w[start : start + 1], boundary, bond)
+        bulk = make(w, bond, bond)
+        last = make(w[:, :, :, end : end + 1], bond, boundary)
+        return cls([first, *[bulk] * (n_sites - 2), last])

@@ -45,7 +45,7 @@ class FusionBlockKey:
         return self.output_tree.coupled
 
 
-@dataclass(frozen=True)
+@dataclass(frozen=True, eq=False)
 class TensorStructure:
     """Ordered legs plus everything derivable from them.  Immutable and hashable."""
 
@@ -55,6 +55,17 @@ class TensorStructure:
         if not self.legs:
             raise ValueError("TensorStructure needs at least one leg")
         object.__setattr__(self, "legs", tuple(self.legs))
+        object.__setattr__(self, "_hash", hash(self.legs))
+
+    def __hash__(self) -> int:
+        return self._hash
+
+    def __eq__(self, other) -> bool:
+        if self is other:
+            return True
+        if not isinstance(other, TensorStructure):
+            return NotImplemented
+        return self._hash == other._hash and self.legs == other.legs
 
     @property
     def provider(self):
