@@ -52,14 +52,19 @@ def time_arm(fn, steps: int, warmup: int):
     return (time.perf_counter() - t0) / steps
 
 
-def restarted_lanczos(op, aa, refine=None, restarts: int = 6, ncv: int = 4):
+def restarted_lanczos(op, aa, refine=None, max_restarts: int = 40, ncv: int = 8, tol: float = 1e-14):
+    """Thick-free restarted Lanczos to convergence of the Ritz value; returns (E, matvecs)."""
     from tenet.network import lanczos
 
-    v = aa
-    e = None
-    for _ in range(restarts):
-        e, v = lanczos(op, v, ncv=ncv, refine=refine)
-    return e
+    v, e, used = aa, None, 0
+    for _ in range(max_restarts):
+        e_new, v = lanczos(op, v, ncv=ncv, refine=refine)
+        used += ncv + (1 if refine else 0)
+        if e is not None and abs(e_new - e) <= tol * max(1.0, abs(e_new)):
+            e = e_new
+            break
+        e = e_new
+    return e, used
 
 
 def cpu_baseline(fl, w1, w2, fr, aa, budget_s: float = 15.0):
@@ -167,14 +172,18 @@ def main() -> int:
                 "flops_per_launch": {str(k): per_step[k][1] for k in per_step}}
 
     # energy agreement: FP32/BF16x3 Krylov + FP64 Rayleigh quotient vs all-FP64
-    e64 = restarted_lanczos(ops_["fp64"], aa)
-    energy = {"E_fp64": e64}
+    e64, n64 = restarted_lanczos(ops_["fp64"], aa)
+    hv = ops_["fp64"](aa)
+    scale = float(tenet.norm(hv)) / float(tenet.norm(aa))  # a lower bound of |H|
+    energy = {"E_fp64": e64, "matvecs_fp64": n64, "operator_scale": scale}
     for p in arms:
         if p == "fp64":
             continue
-        ep = restarted_lanczos(ops_[p], aa, refine=ops_["fp64"])
+        ep, npv = restarted_lanczos(ops_[p], aa, refine=ops_["fp64"])
         energy[f"E_{p}"] = ep
+        energy[f"matvecs_{p}"] = npv
         energy[f"dE_rel_{p}"] = abs(ep - e64) / abs(e64)
+        energy[f"dE_rel_scale_{p}"] = abs(ep - e64) / scale
     log(f"energies {energy}")
 
     cpu = None

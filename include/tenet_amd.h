@@ -57,8 +57,10 @@ extern "C" {
 #define TN_JOB_WORDS 24
 #define TN_MAX_RANK 6
 
-/* GEMM problem table: TN_PROB_WORDS int64 per problem (one output block):
- *   [0] M [1] N [2] c_off [3] ldc [4] seg_begin [5] seg_end [6] c_rows_stride_unused [7] 0
+/* GEMM problem table: TN_PROB_WORDS int64 per problem (one output block, or one K slice
+ * of one when the host splits K):
+ *   [0] M [1] N [2] c_off [3] ldc [4] seg_begin [5] seg_end
+ *   [6] 0 = write to C, 1 = write to the workspace W (split-K partial)  [7] 0
  * Segment table: TN_SEG_WORDS int64 per K segment:
  *   [0] a_off [1] b_off [2] K [3] lda [4] ldb [5] 0
  * Tile table: 4 int32 per output tile: (problem, tile row, tile col, 0). */
@@ -76,14 +78,15 @@ int tn_recouple(int dtype, const void* src, void* dst, const int64_t* jobs,
                 const int64_t* term_src, const double* term_coeff, const int64_t* tiles,
                 int64_t ntiles, void* stream);
 
-/* For every problem p: C_p = sum_s op(A_s) op(B_s) with op = transpose when trans_a/b.
- * A/B/C are base pointers; every offset in the tables is in elements.  bm x bn is the
+/* For every problem p: C_p = sum_s op(A_s) op(B_s) with op = transpose when trans_a/b,
+ * written to C, or to the workspace W when the problem is flagged as a split-K partial.
+ * A/B/C/W are base pointers; every offset in the tables is in elements.  bm x bn is the
  * workgroup tile (one of 32x32, 64x64, 128x64, 64x128, 128x128); grid 0 = one workgroup
  * per tile, otherwise a persistent loop over tiles. */
 int tn_gemm_grouped(int precision, int dtype, int trans_a, int trans_b, const void* A,
-                    const void* B, void* C, const int64_t* problems, const int64_t* segs,
-                    const int32_t* tiles, int64_t ntiles, int bm, int bn, int grid,
-                    void* stream);
+                    const void* B, void* C, void* W, const int64_t* problems,
+                    const int64_t* segs, const int32_t* tiles, int64_t ntiles, int bm, int bn,
+                    int grid, void* stream);
 
 /* out[0] = sum_i w_i x_i y_i (w may be NULL for unit weights), FP64 accumulation,
  * deterministic two-pass reduction; `work` must hold >= 1024 doubles. */
@@ -109,11 +112,13 @@ int tn_cast(int src_dtype, const void* src, int dst_dtype, void* dst, int64_t n,
  *   out[o][r][s] = sum_k coef[o][k] * in[k][r][s]
  * where in/out slabs are addressed by per-job offset lists.  Job table per job (int64):
  *   [0] R [1] S [2] in_begin [3] in_end [4] out_begin [5] out_end [6] coef_off [7] 0
- * in/out lists hold (offset, row_stride) int64 pairs; coef is a dense row-major
- * (nout x nin) double table per job. */
+ * in/out lists hold (offset, row_stride) int64 pairs (at most TN_MPO_MAX_IN inputs per
+ * job); coef holds, per job, nout rows of TN_MPO_MAX_IN coefficients (zero-padded) in the
+ * element type.  Tile table: 2 int64 per tile: (job, block of 1024 positions). */
 #define TN_MPO_JOB_WORDS 8
+#define TN_MPO_MAX_IN 32
 int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t* jobs, int64_t njobs,
-                 const int64_t* in_slabs, const int64_t* out_slabs, const double* coef,
+                 const int64_t* in_slabs, const int64_t* out_slabs, const void* coef,
                  const int64_t* tiles, int64_t ntiles, void* stream);
 
 #ifdef __cplusplus

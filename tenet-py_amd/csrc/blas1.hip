@@ -114,35 +114,48 @@ unsigned grid_for(int64_t n) {
 }
 
 // ---- fused two-site MPO application --------------------------------------------------
-constexpr int kMpoMax = 32;  // max slabs on either side of one job
+// One workgroup owns 1024 positions (r, s) of one slab-pair job.  Every index into the
+// per-thread input array is a compile-time constant (inputs beyond nin are zero and the
+// coefficient rows are zero-padded to kMpoMax columns), so the array lives in VGPRs; the
+// coefficients and slab tables are uniform across the wave and come in through scalar
+// loads.
+constexpr int kMpoMax = 32;   // max input slabs of one job (coefficient row length)
+constexpr int kMpoPer = 4;    // positions per thread
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void mpo_kernel(const T* __restrict__ in, T* __restrict__ out,
                                                        const int64_t* __restrict__ jobs,
                                                        const int64_t* __restrict__ in_slabs,
                                                        const int64_t* __restrict__ out_slabs,
-                                                       const double* __restrict__ coef,
+                                                       const T* __restrict__ coef,
                                                        const int64_t* __restrict__ tiles) {
   const int64_t job = tiles[2 * blockIdx.x];
-  const int64_t e = tiles[2 * blockIdx.x + 1] * kThreads + threadIdx.x;
+  const int64_t base = tiles[2 * blockIdx.x + 1] * (kThreads * kMpoPer);
   const int64_t* jw = jobs + job * TN_MPO_JOB_WORDS;
   const int64_t R = jw[0], S = jw[1];
-  if (e >= R * S) return;
-  const int64_t r = e / S, s = e - (e / S) * S;
   const int nin = static_cast<int>(jw[3] - jw[2]);
   const int nout = static_cast<int>(jw[5] - jw[4]);
   if (nin > kMpoMax) return;  // the host refuses such tables; never index past v[]
   const int64_t* is = in_slabs + 2 * jw[2];
   const int64_t* os = out_slabs + 2 * jw[4];
-  const double* cf = coef + jw[6];
-  double v[kMpoMax];
-#pragma unroll 4
-  for (int k = 0; k < nin; ++k) v[k] = static_cast<double>(in[is[2 * k] + r * is[2 * k + 1] + s]);
-  for (int o = 0; o < nout; ++o) {
-    double acc = 0.0;
-    const double* row = cf + o * nin;
-    for (int k = 0; k < nin; ++k) acc += row[k] * v[k];
-    out[os[2 * o] + r * os[2 * o + 1] + s] = static_cast<T>(acc);
+  const T* cf = coef + jw[6];
+  const uint32_t S32 = static_cast<uint32_t>(S);
+  const int64_t n = R * S;
+  for (int j = 0; j < kMpoPer; ++j) {
+    const int64_t e = base + j * kThreads + threadIdx.x;
+    if (e >= n) break;
+    const uint32_t r = static_cast<uint32_t>(e) / S32;
+    const int64_t s = static_cast<int64_t>(static_cast<uint32_t>(e) - r * S32);
+    T v[kMpoMax];
+#pragma unroll
+    for (int k = 0; k < kMpoMax; ++k) v[k] = k < nin ? in[is[2 * k] + r * is[2 * k + 1] + s] : T(0);
+    for (int o = 0; o < nout; ++o) {
+      const T* row = cf + o * kMpoMax;
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < kMpoMax; ++k) acc += row[k] * v[k];
+      out[os[2 * o] + r * os[2 * o + 1] + s] = acc;
+    }
   }
 }
 
@@ -214,7 +227,7 @@ extern "C" int tn_cast(int src_dtype, const void* src, int dst_dtype, void* dst,
 
 extern "C" int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t* jobs,
                             int64_t njobs, const int64_t* in_slabs, const int64_t* out_slabs,
-                            const double* coef, const int64_t* tiles, int64_t ntiles,
+                            const void* coef, const int64_t* tiles, int64_t ntiles,
                             void* stream) {
   (void)njobs;
   if (ntiles <= 0) return 0;
@@ -223,10 +236,12 @@ extern "C" int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t*
   const unsigned g = static_cast<unsigned>(ntiles);
   if (dtype == TN_F64)
     mpo_kernel<double><<<g, kThreads, 0, s>>>(static_cast<const double*>(in), static_cast<double*>(out),
-                                              jobs, in_slabs, out_slabs, coef, tiles);
+                                              jobs, in_slabs, out_slabs,
+                                              static_cast<const double*>(coef), tiles);
   else if (dtype == TN_F32)
     mpo_kernel<float><<<g, kThreads, 0, s>>>(static_cast<const float*>(in), static_cast<float*>(out),
-                                             jobs, in_slabs, out_slabs, coef, tiles);
+                                             jobs, in_slabs, out_slabs,
+                                             static_cast<const float*>(coef), tiles);
   else
     return tn::fail("tn_mpo_apply: unknown dtype");
   return tn::check_launch("tn_mpo_apply");

@@ -294,39 +294,87 @@ class GemmProgram:
     """A compiled ``tn_gemm_grouped`` launch over a list of block problems.
 
     ``problems``: list of ``(M, N, c_off, ldc, [(a_off, b_off, K, lda, ldb), ...])``.
+
+    When the problems give too few tiles to fill the device, long-K problems are split
+    into K slices whose partial products go to a workspace; one multi-term recouple launch
+    then sums the slices in a fixed order into C (deterministic split-K).  Tiles are
+    issued longest-K first so the tail of the launch is short tiles.
     """
 
+    SPLIT_TARGET_TILES = 1536
+    SPLIT_MIN_K = 256
+
     def __init__(self, problems, a_size, b_size, c_size, trans_a=False, trans_b=False,
-                 tile=None) -> None:
+                 tile=None, split: bool = True) -> None:
         self.trans_a, self.trans_b = bool(trans_a), bool(trans_b)
-        P = len(problems)
-        probs = np.zeros((P, PROB_WORDS), dtype=np.int64)
-        segs = []
-        Ms, Ns, Ks = [], [], []
+        self.a_size, self.b_size, self.c_size = a_size, b_size, c_size
         self.flops = 0
-        for p, (M, N, c_off, ldc, sl) in enumerate(problems):
-            probs[p, :6] = (M, N, c_off, ldc, len(segs), len(segs) + len(sl))
+        for M, N, c_off, ldc, sl in problems:
             if c_off < 0 or c_off + (M - 1) * ldc + N > c_size:
                 raise ValueError("gemm problem writes outside the output buffer")
-            ktot = 0
             for a_off, b_off, K, lda, ldb in sl:
                 a_hi = a_off + ((K - 1) * lda + M if self.trans_a else (M - 1) * lda + K)
                 b_hi = b_off + ((N - 1) * ldb + K if self.trans_b else (K - 1) * ldb + N)
                 if a_off < 0 or b_off < 0 or a_hi > a_size or b_hi > b_size:
                     raise ValueError("gemm segment reads outside its operand buffer")
-                segs.append((a_off, b_off, K, lda, ldb, 0))
-                ktot += K
-            self.flops += 2 * M * N * ktot
-            Ms.append(M)
-            Ns.append(N)
-            Ks.append(max(ktot, 1))
-        self.Ms, self.Ns, self.Ks = Ms, Ns, Ks
-        self.a_size, self.b_size, self.c_size = a_size, b_size, c_size
-        self.nproblems = P
-        self._probs = probs
+                self.flops += 2 * M * N * K
+        Ks0 = [max(sum(sg[2] for sg in sl), 1) for *_, sl in problems]
+        self.tile = tuple(tile) if tile else (
+            choose_tile([p[0] for p in problems], [p[1] for p in problems], Ks0) if problems else (64, 64))
+        bm, bn = self.tile
+        ntiles = sum(-(-p[0] // bm) * -(-p[1] // bn) for p in problems)
+        factor = 1
+        if split and problems and ntiles < self.SPLIT_TARGET_TILES:
+            factor = min(16, -(-self.SPLIT_TARGET_TILES // max(ntiles, 1)))
+        rows, segs, red = [], [], []
+        ws = 0
+        for (M, N, c_off, ldc, sl), kt in zip(problems, Ks0):
+            nsplit = 1
+            if factor > 1 and kt >= self.SPLIT_MIN_K:
+                nsplit = max(1, min(factor, kt // (self.SPLIT_MIN_K // 2)))
+            if nsplit == 1:
+                rows.append((M, N, c_off, ldc, len(segs), len(segs) + len(sl), 0, kt))
+                segs.extend((*sg, 0) for sg in sl)
+                continue
+            chunks = self._split_segments(sl, kt, nsplit)
+            offs = []
+            for ch in chunks:
+                rows.append((M, N, ws, N, len(segs), len(segs) + len(ch), 1, sum(c[2] for c in ch)))
+                segs.extend((*c, 0) for c in ch)
+                offs.append(ws)
+                ws += M * N
+            red.append((c_off, (M, N), (ldc, 1), (N, 1), [(o, 1.0) for o in offs]))
+        self.ws_size = ws
+        self.reduce = RecoupleProgram(red, ws, c_size) if red else None
+        self._rows = rows
+        self.nproblems = len(rows)
+        self._probs = np.asarray([r[:7] + (0,) for r in rows], dtype=np.int64).reshape(-1, PROB_WORDS)
         self._segs = np.asarray(segs, dtype=np.int64).reshape(-1, SEG_WORDS)
+        self.Ms = [r[0] for r in rows]
+        self.Ns = [r[1] for r in rows]
+        self.Ks = [r[7] for r in rows]
         self._by_tile: dict = {}
-        self.tile = tuple(tile) if tile else (choose_tile(Ms, Ns, Ks) if P else (64, 64))
+
+    def _split_segments(self, sl, kt, nsplit):
+        """Cut the K range into ``nsplit`` slices at multiples of 32, splitting segments."""
+        step = -(-kt // nsplit)
+        step = -(-step // 32) * 32
+        chunks, cur, filled = [], [], 0
+        for a_off, b_off, K, lda, ldb in sl:
+            k0 = 0
+            while k0 < K:
+                take = min(K - k0, step - filled)
+                ao = a_off + (k0 * lda if self.trans_a else k0)
+                bo = b_off + (k0 if self.trans_b else k0 * ldb)
+                cur.append((ao, bo, take, lda, ldb))
+                filled += take
+                k0 += take
+                if filled == step:
+                    chunks.append(cur)
+                    cur, filled = [], 0
+        if cur:
+            chunks.append(cur)
+        return chunks
 
     def _tables(self, tile, device):
         d = self._by_tile.get(tile)
@@ -339,8 +387,12 @@ class GemmProgram:
                     continue
                 ii, jj = np.meshgrid(np.arange(tm), np.arange(tn), indexing="ij")
                 rows.append(np.stack([np.full(tm * tn, p), ii.ravel(), jj.ravel(),
-                                      np.zeros(tm * tn)], 1))
-            tiles = np.concatenate(rows).astype(np.int32) if rows else np.zeros((0, 4), np.int32)
+                                      np.full(tm * tn, self.Ks[p])], 1))
+            tiles = np.concatenate(rows) if rows else np.zeros((0, 4), np.int64)
+            # longest K first; ties keep problem order
+            tiles = tiles[np.argsort(-tiles[:, 3], kind="stable")]
+            tiles[:, 3] = 0
+            tiles = tiles.astype(np.int32)
             d = _DeviceCache(probs=self._probs.reshape(-1), segs=self._segs.reshape(-1),
                              tiles=tiles.reshape(-1))
             d.ntiles = len(tiles)
@@ -358,6 +410,7 @@ class GemmProgram:
         prec = precision or gemm_precision(C.dtype)
         tile = tuple(tile) if tile else self.tile
         t, ntiles = self._tables(tile, C.device)
+        W = torch.empty(self.ws_size, dtype=C.dtype, device=C.device) if self.ws_size else C
         _native.check(
             _native.lib().tn_gemm_grouped(
                 _native.PRECISIONS[prec],
@@ -367,6 +420,7 @@ class GemmProgram:
                 A.data_ptr(),
                 B.data_ptr(),
                 C.data_ptr(),
+                W.data_ptr(),
                 t["probs"].data_ptr(),
                 t["segs"].data_ptr(),
                 t["tiles"].data_ptr(),
@@ -377,6 +431,8 @@ class GemmProgram:
                 _native.stream_ptr(),
             )
         )
+        if self.reduce is not None:
+            self.reduce.run(W, C)
 
 
 # ----------------------------------------------------------------------------- contraction

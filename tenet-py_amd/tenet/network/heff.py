@@ -178,13 +178,16 @@ class _FusedChain:
             if len(ilist) > 32:
                 raise ValueError("two-site MPO slab pair has more than 32 input channels")
             R, S = olist[0][3], olist[0][4]
-            cm = np.array([[w12[o[2] + i[2]] for i in ilist] for o in olist], dtype=np.float64).reshape(len(olist), len(ilist))
+            cm = np.zeros((len(olist), 32), dtype=np.float64)
+            for oi, o in enumerate(olist):
+                for ki, i in enumerate(ilist):
+                    cm[oi, ki] = w12[o[2] + i[2]]
             j = len(jobs)
             jobs.append((R, S, len(ins), len(ins) + len(ilist), len(outs), len(outs) + len(olist), len(coef), 0))
             ins.extend((i[0], i[1]) for i in ilist)
             outs.extend((o[0], o[1]) for o in olist)
             coef.extend(cm.reshape(-1).tolist())
-            nt = -(-(R * S) // 256)
+            nt = -(-(R * S) // 1024)
             tiles.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
             for off, rs, _, _, _ in ilist:
                 if off + (R - 1) * rs + S > s1.size:
@@ -199,7 +202,8 @@ class _FusedChain:
             jobs=np.asarray(jobs, dtype=np.int64).reshape(-1),
             ins=np.asarray(ins, dtype=np.int64).reshape(-1),
             outs=np.asarray(outs, dtype=np.int64).reshape(-1),
-            coef=np.asarray(coef, dtype=np.float64),
+            coef64=np.asarray(coef, dtype=np.float64),
+            coef32=np.asarray(coef, dtype=np.float32),
             tiles=tiles.reshape(-1),
         )
         self.mpo_bytes_elems = s1.size + self.t3_size
@@ -228,7 +232,8 @@ class _FusedChain:
         _native.check(_native.lib().tn_mpo_apply(
             _native.TN_F64 if x.dtype == torch.float64 else _native.TN_F32,
             t1.data_ptr(), t3.data_ptr(), tb["jobs"].data_ptr(), self.mpo_njobs,
-            tb["ins"].data_ptr(), tb["outs"].data_ptr(), tb["coef"].data_ptr(),
+            tb["ins"].data_ptr(), tb["outs"].data_ptr(),
+            (tb["coef64"] if x.dtype == torch.float64 else tb["coef32"]).data_ptr(),
             tb["tiles"].data_ptr(), self.mpo_ntiles, _native.stream_ptr()))
         out = torch.empty(self.s_out.size, dtype=x.dtype, device=x.device)
         if events is not None:

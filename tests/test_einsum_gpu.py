@@ -105,3 +105,26 @@ def test_vector_space_ops():
     assert rel_err(to_oracle(c).flat(), oc.flat()) < 1e-14
     d = tenet.divide(a, tenet.norm(a))
     assert abs(float(tenet.norm(d)) - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 5e-5)])
+def test_split_k_long_contraction(prec, tol):
+    """Few output tiles and a long K: the GEMM splits K into slices summed in a fixed order."""
+    import tenet
+    from tenet.engine import contraction_exec
+
+    small, big = {0: 3, 1: 2, -1: 5}, {0: 700, 1: 500, -1: 300}
+    import tenet as tn
+    a = tn.SymmetricTensor.random((leg(small, 1), leg(big, 0)), seed=21, device="cuda")
+    b = tn.SymmetricTensor.random((leg(big, 1), leg(small, 0)), seed=22, device="cuda")
+    oa = O.OTensor.random((oleg(small, 1), oleg(big, 0)), 21)
+    ob = O.OTensor.random((oleg(big, 1), oleg(small, 0)), 22)
+    ex = contraction_exec(a.structure, b.structure, ((1,), (0,)), (0, 1))
+    assert ex.gemm.reduce is not None and ex.gemm.ws_size > 0  # the split path is the one tested
+    ref = O.einsum("ab,bc->ac", oa, ob)
+    if prec != "fp64":
+        tenet.set_gemm_precision(prec)
+        a, b = a.to(dtype=torch.float32), b.to(dtype=torch.float32)
+    out = tenet.einsum("ab,bc->ac", a, b)
+    tenet.set_gemm_precision("fp32")
+    assert rel_err(to_oracle(out).flat(), ref.flat()) < tol
