@@ -308,6 +308,7 @@ class GemmProgram:
                  tile=None, split: bool = True) -> None:
         self.trans_a, self.trans_b = bool(trans_a), bool(trans_b)
         self.a_size, self.b_size, self.c_size = a_size, b_size, c_size
+        self.problems = problems  # unsplit, for merging into batched launches
         self.flops = 0
         for M, N, c_off, ldc, sl in problems:
             if c_off < 0 or c_off + (M - 1) * ldc + N > c_size:

@@ -1,0 +1,180 @@
+"""Many independent two-site H_eff problems in one launch per step.
+
+``BatchedHeff2`` takes the compiled abelian chains of any number of bonds/chains (each a
+``_FusedChain``: GEMM, fused two-site MPO pass, transposed GEMM), packs every instance's
+operands into shared arena buffers and merges their tables with offset shifts, so one
+matvec of the whole batch is three launches: one persistent grouped GEMM over every
+instance's step-1 blocks, one MPO pass over every slab pair, one grouped GEMM over every
+step-3 block.  Uneven sector sizes of all instances share one tile table, sorted
+longest-K first; the tile shape of each GEMM is chosen by the autotuner
+(``tenet.autotune``) from the histogram of its problem sizes.
+
+``synthetic_operator`` builds one XXZ instance directly: random environments made
+Hermitian under the MPO's channel conjugation (``S^+ <-> S^-``), so ``H_eff`` is a real
+symmetric operator without folding a whole chain.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from tenet import _native, ops
+from tenet.engine import GemmProgram, _DeviceCache
+from tenet.network.heff import Heff2
+from tenet.network.models import (
+    BOUNDARY,
+    END,
+    MPO_BOND,
+    PHYS,
+    SM_CH,
+    SP_CH,
+    gaussian_bond,
+    heisenberg_mpo,
+)
+from tenet.network.mps import from_dense
+from tenet.space import IN, OUT, Leg
+from tenet.structure import TensorStructure
+from tenet.tensor import SymmetricTensor
+
+__all__ = ["BatchedHeff2", "synthetic_operator", "batch_workload"]
+
+_BAR = np.array([4, 1, 2, 3, 0])  # MPO channel conjugation: SP (0) <-> SM (4)
+
+
+def _charges(space):
+    return np.concatenate([np.full(m, a.charge) for a, m in space.sectors])
+
+
+def _symmetric_env(bond, left: bool, rng, device):
+    """A Hermitian environment on ``bond``: F[a, x, B] = F[B, bar(x), a] (left) or
+    F[r, y, s] = F[s, bar(y), r] (right), zero outside the U(1) block set."""
+    qb = _charges(bond)
+    qw = _charges(MPO_BOND)
+    D = len(qb)
+    f = rng.standard_normal((D, 5, D))
+    if left:  # (a IN, x OUT, B OUT): x + B = a
+        mask = (qw[None, :, None] + qb[None, None, :]) == qb[:, None, None]
+    else:     # (r OUT, y IN, s IN): r = y + s
+        mask = qb[:, None, None] == (qw[None, :, None] + qb[None, None, :])
+    f *= mask
+    f = 0.5 * (f + np.transpose(f[:, _BAR, :], (2, 1, 0)))
+    legs = ((Leg(bond, IN), Leg(MPO_BOND, OUT), Leg(bond, OUT)) if left
+            else (Leg(bond, OUT), Leg(MPO_BOND, IN), Leg(bond, IN)))
+    t = from_dense(f, legs, device=device)
+    return ops.divide(t, ops.norm(t))
+
+
+def synthetic_operator(n_sites: int, max_dim: int, sigma: float, delta: float, seed: int,
+                       device="cuda"):
+    """``(fl, w_left, w_right, fr, aa)`` of one XXZ two-site H_eff at the centre bond."""
+    rng = np.random.default_rng(seed)
+    c = n_sites // 2 - 1
+    bl = gaussian_bond(c, n_sites, max_dim, sigma)
+    br = gaussian_bond(c + 2, n_sites, max_dim, sigma)
+    h = heisenberg_mpo(n_sites, delta, device=device)
+    fl = _symmetric_env(bl, True, rng, device)
+    fr = _symmetric_env(br, False, rng, device)
+    aa = SymmetricTensor.random((Leg(bl, OUT), Leg(PHYS, OUT), Leg(PHYS, OUT), Leg(br, IN)),
+                                seed=seed, device=device)
+    aa = ops.divide(aa, ops.norm(aa))
+    return fl, h[c], h[c + 1], fr, aa
+
+
+def batch_workload(count: int = 256, n_sites: int = 32, d_min: int = 64, d_max: int = 1024,
+                   sigma: float = 6.0, seed: int = 0, device="cuda"):
+    """Delta ~ U[0, 2], D log-uniform in [d_min, d_max], from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    deltas = rng.uniform(0.0, 2.0, count)
+    dims = np.exp(rng.uniform(math.log(d_min), math.log(d_max), count)).astype(int)
+    return [synthetic_operator(n_sites, int(d), sigma, float(dl), seed + 1 + i, device)
+            for i, (d, dl) in enumerate(zip(dims, deltas))], dims, deltas
+
+
+def _shift_problems(problems, c_shift, a_shift, b_shift):
+    return [(M, N, c_off + c_shift, ldc, [(a + a_shift, b + b_shift, K, lda, ldb) for a, b, K, lda, ldb in segs])
+            for M, N, c_off, ldc, segs in problems]
+
+
+class BatchedHeff2:
+    """One matvec over a batch of abelian two-site H_eff instances."""
+
+    def __init__(self, instances, precision: str = "fp32", tiles: dict | None = None) -> None:
+        dt = torch.float64 if precision == "fp64" else torch.float32
+        self.precision, self.dtype = precision, dt
+        self.ops = [Heff2(fl, w1, w2, fr, precision) for fl, w1, w2, fr, _ in instances]
+        self.structures = [inst[4].structure for inst in instances]
+        chains = [op.chain(s) for op, s in zip(self.ops, self.structures)]
+        for ch in chains:
+            if not hasattr(ch, "mpo_host"):
+                raise ValueError("BatchedHeff2 needs abelian (fused-path) operators")
+            if ch.ex1.a.program is not None or ch.ex1.b.program is not None or ch.ex1.c_program is not None:
+                raise ValueError("BatchedHeff2 needs in-place step-1 operands")
+        dev = instances[0][4].device
+        # arenas
+        def offsets(sizes):
+            o = np.zeros(len(sizes) + 1, dtype=np.int64)
+            np.cumsum(sizes, out=o[1:])
+            return o
+        self.x_off = offsets([s.size for s in self.structures])
+        fr_off = offsets([op._statics[0][0].structure.size for op in self.ops])
+        fl_off = offsets([op._fl_w.structure.size for op in self.ops])
+        t1_off = offsets([ch.t1_size for ch in chains])
+        t3_off = offsets([ch.t3_size for ch in chains])
+        self.fr = torch.cat([op._statics[0][0].data for op in self.ops]).to(dev)
+        self.fl = torch.cat([op._fl_w.data for op in self.ops]).to(dev)
+        self.t1_size, self.t3_size = int(t1_off[-1]), int(t3_off[-1])
+        self.x_size = int(self.x_off[-1])
+        p1, p4 = [], []
+        jobs, ins, outs, coef, tl = [], [], [], [], []
+        for i, ch in enumerate(chains):
+            p1 += _shift_problems(ch.ex1.gemm.problems, int(t1_off[i]), int(self.x_off[i]), int(fr_off[i]))
+            p4 += _shift_problems(ch.gemm4.problems, int(self.x_off[i]), int(fl_off[i]), int(t3_off[i]))
+            j0, i0, o0, c0 = len(jobs), len(ins), len(outs), len(coef)
+            cj, ci, co, cc = ch.mpo_host
+            for R, S, ib, ie, ob, oe, cf, _ in cj:
+                jobs.append((R, S, ib + i0, ie + i0, ob + o0, oe + o0, cf + c0, 0))
+                nt = -(-(R * S) // 1024)
+                tl.append(np.stack([np.full(nt, len(jobs) - 1, np.int64), np.arange(nt, dtype=np.int64)], 1))
+            ins += [(off + int(t1_off[i]), rs) for off, rs in ci]
+            outs += [(off + int(t3_off[i]), rs) for off, rs in co]
+            coef += cc
+        tune = tiles or {}
+        self.gemm1 = GemmProgram(p1, self.x_size, int(fr_off[-1]), self.t1_size, tile=tune.get("gemm1"))
+        self.gemm4 = GemmProgram(p4, int(fl_off[-1]), self.t3_size, self.x_size, trans_a=True,
+                                 tile=tune.get("gemm4"))
+        tl = np.concatenate(tl) if tl else np.zeros((0, 2), np.int64)
+        self.mpo_njobs, self.mpo_ntiles = len(jobs), len(tl)
+        self.mpo = _DeviceCache(jobs=np.asarray(jobs, np.int64).reshape(-1),
+                                ins=np.asarray(ins, np.int64).reshape(-1),
+                                outs=np.asarray(outs, np.int64).reshape(-1),
+                                coef=np.asarray(coef, np.float64 if dt == torch.float64 else np.float32),
+                                tiles=tl.reshape(-1))
+        self.flops = self.gemm1.flops + self.gemm4.flops
+        self.count = len(instances)
+
+    def pack(self, vectors) -> torch.Tensor:
+        """One arena buffer (working dtype) holding every instance's input vector."""
+        return torch.cat([v.data.to(self.dtype) for v in vectors])
+
+    def unpack(self, x: torch.Tensor) -> list:
+        return [SymmetricTensor(s, x[int(a):int(b)]) for s, a, b in zip(self.structures, self.x_off[:-1], self.x_off[1:])]
+
+    def apply(self, x: torch.Tensor, tiles: dict | None = None) -> torch.Tensor:
+        if x.dtype != self.dtype or x.numel() != self.x_size:
+            raise ValueError("BatchedHeff2.apply: the arena does not match the batch")
+        t = tiles or {}
+        t1 = torch.empty(self.t1_size, dtype=x.dtype, device=x.device)
+        self.gemm1.run(x, self.fr, t1, self.precision, tile=t.get("gemm1"))
+        t3 = torch.empty(self.t3_size, dtype=x.dtype, device=x.device)
+        tb = self.mpo.get(x.device)
+        _native.check(_native.lib().tn_mpo_apply(
+            _native.TN_F64 if x.dtype == torch.float64 else _native.TN_F32,
+            t1.data_ptr(), t3.data_ptr(), tb["jobs"].data_ptr(), self.mpo_njobs,
+            tb["ins"].data_ptr(), tb["outs"].data_ptr(), tb["coef"].data_ptr(),
+            tb["tiles"].data_ptr(), self.mpo_ntiles, _native.stream_ptr()))
+        out = torch.empty(self.x_size, dtype=x.dtype, device=x.device)
+        self.gemm4.run(self.fl, t3, out, self.precision, tile=t.get("gemm4"))
+        return out

@@ -195,6 +195,8 @@ class _FusedChain:
             for off, rs, _, _, _ in olist:
                 if off + (R - 1) * rs + S > self.t3_size:
                     raise ValueError("MPO pass writes outside t3")
+        self.mpo_host = (jobs, ins, outs, coef)
+        self.t1_size = s1.size
         self.mpo_njobs = len(jobs)
         tiles = np.concatenate(tiles) if tiles else np.zeros((0, 2), np.int64)
         self.mpo_ntiles = len(tiles)

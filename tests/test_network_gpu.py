@@ -101,3 +101,30 @@ def test_dmrg_xx_chain_matches_closed_form():
     psi = random_chain(L, 16, 3.0, seed=2, device="cuda")
     out = dmrg_(psi, heisenberg_mpo(L, delta=0.0, device="cuda"), chi=48, max_sweeps=12)
     assert abs(out.energy - ed.xx_energy(L)) < 1e-10 * abs(ed.xx_energy(L))
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+def test_batched_heff2_matches_individual_operators(prec, tol):
+    from tenet.network import Heff2
+    from tenet.network.batched import BatchedHeff2, synthetic_operator
+
+    insts = [synthetic_operator(16, d, 3.0, dl, s, device="cuda") for d, dl, s in ((40, 0.5, 1), (70, 1.3, 2), (24, 1.9, 3))]
+    b = BatchedHeff2(insts, prec)
+    out = b.unpack(b.apply(b.pack([i[4] for i in insts])))
+    for inst, got in zip(insts, out):
+        ref = Heff2(*inst[:4], "fp64")(inst[4])
+        assert got.structure == ref.structure
+        assert rel_err(got.data.double().cpu().numpy(), ref.data.cpu().numpy()) < tol
+
+
+def test_synthetic_operator_is_symmetric():
+    from tenet.network import Heff2
+    from tenet.network.batched import synthetic_operator
+    import tenet
+
+    fl, w1, w2, fr, aa = synthetic_operator(16, 40, 3.0, 0.7, 5, device="cuda")
+    op = Heff2(fl, w1, w2, fr, "fp64")
+    x = tenet.SymmetricTensor.random(aa.legs, seed=8, device="cuda")
+    y = tenet.SymmetricTensor.random(aa.legs, seed=9, device="cuda")
+    a, b = float(tenet.inner(x, op(y))), float(tenet.inner(op(x), y))
+    assert abs(a - b) < 1e-12 * max(abs(a), 1e-300)
