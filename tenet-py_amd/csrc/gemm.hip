@@ -123,181 +123,299 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(
   int logical = blockIdx.x;
   if ((G & 7) == 0) logical = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
 
-  for (int t = logical; t < ntiles; t += G) {
+  // Tile descriptor of the tile being fetched (may run one tile ahead of the one computed).
+  struct Tile {
+    int m0, n0, M, N;
+    int64_t c_off, ldc, sb, se;
+    int flag;
+  };
+  auto tile_of = [&](int t) -> Tile {
     const int p = tiles[4 * t];
-    const int m0 = tiles[4 * t + 1] * BM;
-    const int n0 = tiles[4 * t + 2] * BN;
     const int64_t* pw = probs + static_cast<int64_t>(p) * TN_PROB_WORDS;
-    const int M = static_cast<int>(pw[0]);
-    const int N = static_cast<int>(pw[1]);
-    const int64_t c_off = pw[2];
-    const int64_t ldc = pw[3];
-    const int64_t sb = pw[4], se = pw[5];
-    T* __restrict__ out = pw[6] ? W : C;
-
-    Acc acc[SM][SN];
+    return Tile{tiles[4 * t + 1] * BM, tiles[4 * t + 2] * BN, static_cast<int>(pw[0]),
+                static_cast<int>(pw[1]), pw[2], pw[3], pw[4], pw[5], static_cast<int>(pw[6])};
+  };
+  {
+    // ---- operand staging in quads: 4 consecutive k of one row (m of A, n of B) ----
+    // A quad is fetched with one 16-byte buffer load when k is contiguous in memory, or
+    // with 4 scalar loads (coalesced across lanes along m/n) when it is not; either way it
+    // is written to LDS with one conflict-free 16-byte store.  Offsets inside a segment
+    // are computed once per segment; the K-step advance moves the descriptor base
+    // (scalar); elements outside the problem read through an offset past the
+    // descriptor's range, for which the hardware returns zero.
+    using V4 = T __attribute__((ext_vector_type(4)));
+    constexpr int QK = BK / 4;  // quads per row of a K step
+    constexpr int QA = (BM * QK + kThreads - 1) / kThreads;
+    constexpr int QB = (BN * QK + kThreads - 1) / kThreads;
+    constexpr bool QA_EXACT = BM * QK % kThreads == 0;  // else some threads own no A quad
+    constexpr bool QB_EXACT = BN * QK % kThreads == 0;
+    V4 ra[QA], rb[QB];
+    uint32_t ao[QA], bo[QB];
+    bool arow[QA], brow[QB];
+    int akq[QA], bkq[QB];
+    int am[QA], bn_[QB];
+    uint32_t astr = 0, bstr = 0;  // byte stride between the 4 k of a quad when not contiguous
+    int64_t fseg = -1;
+    int ftile = -1;
+    auto offsets = [&](const Tile& tl, const Seg& sg, int64_t sid) {
+      fseg = sid;
+      const int m0 = tl.m0, n0 = tl.n0, M = tl.M, N = tl.N;
+      astr = static_cast<uint32_t>(sg.lda * sizeof(T));
+      bstr = static_cast<uint32_t>(sg.ldb * sizeof(T));
 #pragma unroll
-    for (int i = 0; i < SM; ++i)
-#pragma unroll
-      for (int j = 0; j < SN; ++j) acc[i][j] = Acc{0, 0, 0, 0};
-
-    T ra[LA], rb[LB];
-    auto fetch = [&](const Seg& sg, int k0) {
-      const T* Ap = A + sg.a_off;
-      const T* Bp = B + sg.b_off;
-#pragma unroll
-      for (int r = 0; r < LA; ++r) {
-        const int idx = r * kThreads + tid;
-        int m, k;
-        if (!TA) { k = idx % BK; m = idx / BK; } else { m = idx % BM; k = idx / BM; }
-        const int gm = m0 + m, gk = k0 + k;
-        ra[r] = (gm < M && gk < sg.K) ? (TA ? Ap[gk * sg.lda + gm] : Ap[gm * sg.lda + gk]) : T(0);
+      for (int r = 0; r < QA; ++r) {
+        const int q = r * kThreads + tid;
+        int m, kq;
+        if (!TA) { m = q / QK; kq = (q % QK) * 4; } else { m = q % BM; kq = (q / BM) * 4; }
+        const bool live = QA_EXACT || q < BM * QK;
+        if (!live) { m = 0; kq = 0; }
+        am[r] = live ? m : -1;
+        akq[r] = kq;
+        arow[r] = live && m0 + m < M;
+        const int64_t e = TA ? (int64_t)kq * sg.lda + (m0 + m) : (int64_t)(m0 + m) * sg.lda + kq;
+        ao[r] = static_cast<uint32_t>(e * sizeof(T));
       }
 #pragma unroll
-      for (int r = 0; r < LB; ++r) {
-        const int idx = r * kThreads + tid;
-        int n, k;
-        if (!TB) { n = idx % BN; k = idx / BN; } else { k = idx % BK; n = idx / BK; }
-        const int gn = n0 + n, gk = k0 + k;
-        rb[r] = (gn < N && gk < sg.K) ? (TB ? Bp[gn * sg.ldb + gk] : Bp[gk * sg.ldb + gn]) : T(0);
+      for (int r = 0; r < QB; ++r) {
+        const int q = r * kThreads + tid;
+        int n, kq;
+        if (TB) { n = q / QK; kq = (q % QK) * 4; } else { n = q % BN; kq = (q / BN) * 4; }
+        const bool live = QB_EXACT || q < BN * QK;
+        if (!live) { n = 0; kq = 0; }
+        bn_[r] = live ? n : -1;
+        bkq[r] = kq;
+        brow[r] = live && n0 + n < N;
+        const int64_t e = TB ? (int64_t)(n0 + n) * sg.ldb + kq : (int64_t)kq * sg.ldb + (n0 + n);
+        bo[r] = static_cast<uint32_t>(e * sizeof(T));
       }
+    };
+    auto load1 = [&](__amdgpu_buffer_rsrc_t d, uint32_t off) -> T {
+      if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(d, off, 0, 0));
+      else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(d, off, 0, 0));
+    };
+    // quad of 4 consecutive k starting at byte offset `off`, k-contiguous in memory
+    auto load_contig = [&](__amdgpu_buffer_rsrc_t d, uint32_t off, bool row_ok, int kq, int kleft) -> V4 {
+      V4 v;
+      if (row_ok && kq + 3 < kleft) {
+        if constexpr (sizeof(T) == 8) {
+          typedef double d2 __attribute__((ext_vector_type(2)));
+          const d2 lo = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(d, off, 0, 0));
+          const d2 hi = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(d, off + 16, 0, 0));
+          v = V4{lo[0], lo[1], hi[0], hi[1]};
+        } else {
+          v = __builtin_bit_cast(V4, __builtin_amdgcn_raw_buffer_load_b128(d, off, 0, 0));
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          v[j] = load1(d, (row_ok && kq + j < kleft) ? off + j * (uint32_t)sizeof(T) : 0x80000000u);
+      }
+      return v;
+    };
+    // quad of 4 k values strided by `str` bytes (lanes run along m/n)
+    auto load_strided = [&](__amdgpu_buffer_rsrc_t d, uint32_t off, uint32_t str, bool row_ok, int kq, int kleft) -> V4 {
+      V4 v;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = load1(d, (row_ok && kq + j < kleft) ? off + j * str : 0x80000000u);
+      return v;
+    };
+    auto fetch = [&](int t, const Tile& tl, const Seg& sg, int64_t sid, int k0) {
+      if (sid != fseg || t != ftile) {
+        ftile = t;
+        offsets(tl, sg, sid);
+      }
+      const T* Ap = A + sg.a_off + (TA ? (int64_t)k0 * sg.lda : k0);
+      const T* Bp = B + sg.b_off + (TB ? k0 : (int64_t)k0 * sg.ldb);
+      const __amdgpu_buffer_rsrc_t ra_d = __builtin_amdgcn_make_buffer_rsrc((void*)Ap, (short)0, 0x7fffffff, 0x00020000);
+      const __amdgpu_buffer_rsrc_t rb_d = __builtin_amdgcn_make_buffer_rsrc((void*)Bp, (short)0, 0x7fffffff, 0x00020000);
+      const int kleft = sg.K - k0;
+#pragma unroll
+      for (int r = 0; r < QA; ++r)
+        ra[r] = TA ? load_strided(ra_d, ao[r], astr, arow[r], akq[r], kleft)
+                   : load_contig(ra_d, ao[r], arow[r], akq[r], kleft);
+#pragma unroll
+      for (int r = 0; r < QB; ++r)
+        rb[r] = TB ? load_contig(rb_d, bo[r], brow[r], bkq[r], kleft)
+                   : load_strided(rb_d, bo[r], bstr, brow[r], bkq[r], kleft);
     };
     auto stash = [&](int buf) {
 #pragma unroll
-      for (int r = 0; r < LA; ++r) {
-        const int idx = r * kThreads + tid;
-        int m, k;
-        if (!TA) { k = idx % BK; m = idx / BK; } else { m = idx % BM; k = idx / BM; }
+      for (int r = 0; r < QA; ++r) {
+        if (!QA_EXACT && am[r] < 0) continue;
         if constexpr (PREC == TN_PREC_BF16X3) {
-          const unsigned short h = bf16_rne(ra[r]);
-          sm.ah[buf][m][k] = h;
-          sm.al[buf][m][k] = bf16_rne(ra[r] - bf16_to_f32(h));
+          typedef unsigned short u4 __attribute__((ext_vector_type(4)));
+          u4 h, l;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            h[j] = bf16_rne(ra[r][j]);
+            l[j] = bf16_rne(ra[r][j] - bf16_to_f32(h[j]));
+          }
+          *reinterpret_cast<u4*>(&sm.ah[buf][am[r]][akq[r]]) = h;
+          *reinterpret_cast<u4*>(&sm.al[buf][am[r]][akq[r]]) = l;
         } else {
-          sm.a[buf][m][k] = ra[r];
+          *reinterpret_cast<V4*>(&sm.a[buf][am[r]][akq[r]]) = ra[r];
         }
       }
 #pragma unroll
-      for (int r = 0; r < LB; ++r) {
-        const int idx = r * kThreads + tid;
-        int n, k;
-        if (!TB) { n = idx % BN; k = idx / BN; } else { k = idx % BK; n = idx / BK; }
+      for (int r = 0; r < QB; ++r) {
+        if (!QB_EXACT && bn_[r] < 0) continue;
         if constexpr (PREC == TN_PREC_BF16X3) {
-          const unsigned short h = bf16_rne(rb[r]);
-          sm.bh[buf][n][k] = h;
-          sm.bl[buf][n][k] = bf16_rne(rb[r] - bf16_to_f32(h));
+          typedef unsigned short u4 __attribute__((ext_vector_type(4)));
+          u4 h, l;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            h[j] = bf16_rne(rb[r][j]);
+            l[j] = bf16_rne(rb[r][j] - bf16_to_f32(h[j]));
+          }
+          *reinterpret_cast<u4*>(&sm.bh[buf][bn_[r]][bkq[r]]) = h;
+          *reinterpret_cast<u4*>(&sm.bl[buf][bn_[r]][bkq[r]]) = l;
         } else {
-          sm.b[buf][n][k] = rb[r];
+          *reinterpret_cast<V4*>(&sm.b[buf][bn_[r]][bkq[r]]) = rb[r];
         }
       }
     };
 
-    int64_t s = sb;
-    int k0 = 0;
-    Seg sg{};
-    if (s < se) {
-      sg = load_seg(segs, s);
-      fetch(sg, 0);
-    }
     int buf = 0;
-    while (s < se) {
-      stash(buf);
-      __syncthreads();
-      // advance and prefetch the next K step while this one computes
-      int64_t ns = s;
-      int nk = k0 + BK;
-      Seg nsg = sg;
-      if (nk >= sg.K) {
-        ++ns;
-        nk = 0;
-        if (ns < se) nsg = load_seg(segs, ns);
+    int t = logical;
+    Tile cur{};
+    // fetch position: tile, segment, k0 of the data sitting in the prefetch registers
+    int pt = -1;
+    int64_t ps = 0;
+    int pk = 0;
+    Seg psg{};
+    if (t < ntiles) {
+      cur = tile_of(t);
+      if (cur.sb < cur.se) {
+        psg = load_seg(segs, cur.sb);
+        fetch(t, cur, psg, cur.sb, 0);
+        pt = t; ps = cur.sb; pk = 0;
       }
-      if (ns < se) fetch(nsg, nk);
-
-      if constexpr (PREC == TN_PREC_BF16X3) {
-        const int kq = g * 8;
-        bf16x8 ah[SM], al[SM];
-#pragma unroll
-        for (int i = 0; i < SM; ++i) {
-          const int row = wm0 + i * 16 + r16;
-          ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.ah[buf][row][kq]));
-          al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.al[buf][row][kq]));
-        }
-#pragma unroll
-        for (int j = 0; j < SN; ++j) {
-          const int col = wn0 + j * 16 + r16;
-          const bf16x8 bh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.bh[buf][col][kq]));
-          const bf16x8 bl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.bl[buf][col][kq]));
-#pragma unroll
-          for (int i = 0; i < SM; ++i) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
-          }
-        }
-      } else if constexpr (PREC == TN_PREC_FP32) {
-#pragma unroll
-        for (int h = 0; h < BK / 16; ++h) {
-          const int kq = h * 16 + g * 4;
-          f32x4 a4[SM], b4[SN];
-#pragma unroll
-          for (int i = 0; i < SM; ++i) a4[i] = *reinterpret_cast<const f32x4*>(&sm.a[buf][wm0 + i * 16 + r16][kq]);
-#pragma unroll
-          for (int j = 0; j < SN; ++j) b4[j] = *reinterpret_cast<const f32x4*>(&sm.b[buf][wn0 + j * 16 + r16][kq]);
-#pragma unroll
-          for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-            for (int i = 0; i < SM; ++i)
-#pragma unroll
-              for (int j = 0; j < SN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[i][tt], b4[j][tt], acc[i][j], 0, 0, 0);
-        }
-      } else {
-        const int kq = g * 4;
-        f64x2 a0[SM], a1[SM], b0[SN], b1[SN];
-#pragma unroll
-        for (int i = 0; i < SM; ++i) {
-          const double* pa = &sm.a[buf][wm0 + i * 16 + r16][kq];
-          a0[i] = *reinterpret_cast<const f64x2*>(pa);
-          a1[i] = *reinterpret_cast<const f64x2*>(pa + 2);
-        }
-#pragma unroll
-        for (int j = 0; j < SN; ++j) {
-          const double* pb = &sm.b[buf][wn0 + j * 16 + r16][kq];
-          b0[j] = *reinterpret_cast<const f64x2*>(pb);
-          b1[j] = *reinterpret_cast<const f64x2*>(pb + 2);
-        }
-#pragma unroll
-        for (int i = 0; i < SM; ++i)
-#pragma unroll
-          for (int j = 0; j < SN; ++j) {
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[i][0], b0[j][0], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[i][1], b0[j][1], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[i][0], b1[j][0], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[i][1], b1[j][1], acc[i][j], 0, 0, 0);
-          }
-      }
-      buf ^= 1;
-      s = ns;
-      k0 = nk;
-      sg = nsg;
     }
-    // ---- epilogue ----
+    while (t < ntiles) {
+      Acc acc[SM][SN];
 #pragma unroll
-    for (int i = 0; i < SM; ++i)
+      for (int i = 0; i < SM; ++i)
 #pragma unroll
-      for (int j = 0; j < SN; ++j) {
-        const int col = n0 + wn0 + j * 16 + r16;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          // v_mfma_f64_16x16x4 spreads rows over registers (row 4r + l/16); the FP32
-          // accumulators keep four consecutive rows per lane group (row 4(l/16) + r)
-          const int rr = PREC == TN_PREC_FP64 ? r * 4 + g : g * 4 + r;
-          const int row = m0 + wm0 + i * 16 + rr;
-          if (row < M && col < N) out[c_off + row * ldc + col] = acc[i][j][r];
+        for (int j = 0; j < SN; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+      if (cur.sb < cur.se && pt != t) {  // nothing prefetched for this tile yet
+        psg = load_seg(segs, cur.sb);
+        fetch(t, cur, psg, cur.sb, 0);
+        pt = t; ps = cur.sb; pk = 0;
+      }
+      const int tn = t + G;
+      const Tile nxt = tn < ntiles ? tile_of(tn) : Tile{};
+      while (pt == t) {
+        stash(buf);
+        __syncthreads();
+        // advance the fetch position: next K step of this tile, else the next tile's first
+        int64_t ns = ps;
+        int nk = pk + BK;
+        Seg nsg = psg;
+        if (nk >= psg.K) {
+          ++ns;
+          nk = 0;
+          if (ns < cur.se) nsg = load_seg(segs, ns);
+        }
+        if (ns < cur.se) {
+          fetch(t, cur, nsg, ns, nk);
+          ps = ns; pk = nk; psg = nsg;
+        } else if (tn < ntiles && nxt.sb < nxt.se) {
+          nsg = load_seg(segs, nxt.sb);
+          fetch(tn, nxt, nsg, nxt.sb, 0);
+          pt = tn; ps = nxt.sb; pk = 0; psg = nsg;
+        } else {
+          pt = -1;
+        }
+        if constexpr (PREC == TN_PREC_BF16X3) {
+          const int kq = g * 8;
+          bf16x8 ah[SM], al[SM];
+  #pragma unroll
+          for (int i = 0; i < SM; ++i) {
+            const int row = wm0 + i * 16 + r16;
+            ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.ah[buf][row][kq]));
+            al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.al[buf][row][kq]));
+          }
+  #pragma unroll
+          for (int j = 0; j < SN; ++j) {
+            const int col = wn0 + j * 16 + r16;
+            const bf16x8 bh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.bh[buf][col][kq]));
+            const bf16x8 bl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.bl[buf][col][kq]));
+  #pragma unroll
+            for (int i = 0; i < SM; ++i) {
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+            }
+          }
+        } else if constexpr (PREC == TN_PREC_FP32) {
+  #pragma unroll
+          for (int h = 0; h < BK / 16; ++h) {
+            const int kq = h * 16 + g * 4;
+            f32x4 a4[SM], b4[SN];
+  #pragma unroll
+            for (int i = 0; i < SM; ++i) a4[i] = *reinterpret_cast<const f32x4*>(&sm.a[buf][wm0 + i * 16 + r16][kq]);
+  #pragma unroll
+            for (int j = 0; j < SN; ++j) b4[j] = *reinterpret_cast<const f32x4*>(&sm.b[buf][wn0 + j * 16 + r16][kq]);
+  #pragma unroll
+            for (int tt = 0; tt < 4; ++tt)
+  #pragma unroll
+              for (int i = 0; i < SM; ++i)
+  #pragma unroll
+                for (int j = 0; j < SN; ++j)
+                  acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[i][tt], b4[j][tt], acc[i][j], 0, 0, 0);
+          }
+        } else {
+          const int kq = g * 4;
+          f64x2 a0[SM], a1[SM], b0[SN], b1[SN];
+  #pragma unroll
+          for (int i = 0; i < SM; ++i) {
+            const double* pa = &sm.a[buf][wm0 + i * 16 + r16][kq];
+            a0[i] = *reinterpret_cast<const f64x2*>(pa);
+            a1[i] = *reinterpret_cast<const f64x2*>(pa + 2);
+          }
+  #pragma unroll
+          for (int j = 0; j < SN; ++j) {
+            const double* pb = &sm.b[buf][wn0 + j * 16 + r16][kq];
+            b0[j] = *reinterpret_cast<const f64x2*>(pb);
+            b1[j] = *reinterpret_cast<const f64x2*>(pb + 2);
+          }
+  #pragma unroll
+          for (int i = 0; i < SM; ++i)
+  #pragma unroll
+            for (int j = 0; j < SN; ++j) {
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[i][0], b0[j][0], acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[i][1], b0[j][1], acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[i][0], b1[j][0], acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[i][1], b1[j][1], acc[i][j], 0, 0, 0);
+            }
+        }
+        buf ^= 1;
+      }
+      {
+        const int m0 = cur.m0, n0 = cur.n0, M = cur.M, N = cur.N;
+        const int64_t c_off = cur.c_off, ldc = cur.ldc;
+        T* __restrict__ out = cur.flag ? W : C;
+      // ---- epilogue ----
+  #pragma unroll
+      for (int i = 0; i < SM; ++i)
+  #pragma unroll
+        for (int j = 0; j < SN; ++j) {
+          const int col = n0 + wn0 + j * 16 + r16;
+  #pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            // v_mfma_f64_16x16x4 spreads rows over registers (row 4r + l/16); the FP32
+            // accumulators keep four consecutive rows per lane group (row 4(l/16) + r)
+            const int rr = PREC == TN_PREC_FP64 ? r * 4 + g : g * 4 + r;
+            const int row = m0 + wm0 + i * 16 + rr;
+            if (row < M && col < N) out[c_off + row * ldc + col] = acc[i][j][r];
+          }
         }
       }
-    // the next tile's first stash reuses buffer 0/1: make sure every wave is done reading
-    __syncthreads();
+      t = tn;
+      cur = nxt;
+    }
   }
 }
 

@@ -272,6 +272,31 @@ def scatter_map(S: TensorStructure, D: TensorStructure, perm, terms):
 # ----------------------------------------------------------------------------- gemm
 
 _TILES = ((128, 128), (128, 64), (64, 128), (64, 64), (32, 32))
+_CUS: dict = {}
+
+
+def device_cus() -> int:
+    """Compute units of the current device (queried once through the C ABI)."""
+    if "n" not in _CUS:
+        import ctypes
+
+        n = ctypes.c_int(0)
+        _native.check(_native.lib().tn_device_cus(ctypes.byref(n)))
+        _CUS["n"] = int(n.value)
+    return _CUS["n"]
+
+
+def resident_workgroups(precision: str, tile) -> int:
+    """Workgroups of the GEMM kernel resident at once on the device: the LDS footprint of
+    the double-buffered K step bounds workgroups per CU (160 KiB of LDS per CU), capped at
+    8 (4 waves each, 8 waves per SIMD) and at 2 for the 128x128 tile (register bound)."""
+    bm, bn = tile
+    per_row = 320 if precision == "bf16x3" else 288
+    lds = (bm + bn) * per_row
+    per_cu = max(1, min(8, (160 * 1024) // lds))
+    if bm * bn >= 128 * 128:
+        per_cu = min(per_cu, 2)
+    return device_cus() * per_cu
 
 
 def choose_tile(Ms, Ns, Ks) -> tuple:
@@ -411,6 +436,12 @@ class GemmProgram:
         prec = precision or gemm_precision(C.dtype)
         tile = tuple(tile) if tile else self.tile
         t, ntiles = self._tables(tile, C.device)
+        if grid < 0:
+            # persistent: every resident workgroup walks the tile list, prefetching the
+            # next tile's first K step while it finishes the current one.  Measured slower
+            # than one workgroup per tile on uneven sector sizes (static assignment loses
+            # the dispatcher's dynamic balancing), so it is opt-in.
+            grid = min(ntiles, resident_workgroups(prec, tile))
         W = torch.empty(self.ws_size, dtype=C.dtype, device=C.device) if self.ws_size else C
         _native.check(
             _native.lib().tn_gemm_grouped(
