@@ -26,7 +26,7 @@ import torch
 from tenet import linalg, ops
 from tenet.tensor import SymmetricTensor
 
-__all__ = ["Sweep", "DMRG_out", "lanczos", "sweep_", "dmrg_", "spectrum"]
+__all__ = ["Sweep", "DMRG_out", "lanczos", "sweep_", "sweep_steps", "dmrg_", "dmrg_steps", "spectrum"]
 
 
 class Sweep(NamedTuple):
@@ -125,52 +125,72 @@ def _pair(left, right):
     return ops.einsum("apx,xqr->apqr", left, right)
 
 
-def sweep_(psi, h, env, schmidt: dict, *, chi: int, cutoff: float, ncv: int = 3,
-           noise: float = 0.0, noise_type: str = "wavefunction", orthogonal_to=(), seed: int = 0,
-           precision: str = "fp64", refine: bool = True):
-    """One left-to-right then right-to-left two-site sweep; ``psi`` and ``env`` mutate.
+def _bond_step(psi, env, schmidt, n, forward, *, chi, cutoff, ncv, noise, seed, precision,
+               refine, orthogonal_to=()):
+    """One bond of a two-site sweep: pair, Lanczos, split, write back, update environments.
+    Returns (energy, discarded weight)."""
+    aa = _pair(psi[n], psi[n + 1])
+    op = env.operator(n, precision)
+    ref = env.operator(n, "fp64") if (precision != "fp64" and refine) else None
+    energy, aa = lanczos(op, aa, ncv=ncv, refine=ref)
+    if noise:
+        r = SymmetricTensor.random(aa.legs, seed=seed + n, device=aa.device)
+        aa = ops.add(aa, ops.multiply(r, float(noise * ops.norm(aa) / ops.norm(r))))
+        aa = ops.divide(aa, ops.norm(aa))
+    u, s, vh = linalg.svd_truncated(aa, ((0, 1), (2, 3)), max_bond=chi, cutoff=cutoff)
+    norm_s = float(ops.norm(s))
+    dw = 1.0 - (norm_s / float(ops.norm(aa))) ** 2
+    s = ops.divide(s, norm_s)
+    psi[n + 1] = vh
+    if forward:
+        psi[n] = u
+        psi[n + 1] = ops.einsum("xy,yqr->xqr", s, psi[n + 1])
+    else:
+        psi[n] = ops.einsum("apx,xy->apy", u, s)
+    schmidt[n] = spectrum(s)
+    psi.center = n + 1 if forward else n
+    for e in (env, *orthogonal_to):
+        e.clear_(n, n + 1)
+        if forward:
+            e.update_(n, to="last")
+        else:
+            e.update_(n + 1, to="first")
+    return energy, dw
 
-    ``precision`` selects the H_eff arithmetic (``fp64``, ``fp32``, ``bf16x3``); with a
-    reduced precision and ``refine`` the energy of every bond is the FP64 Rayleigh quotient.
-    """
+
+def sweep_steps(psi, h, env, schmidt: dict, *, chi: int, cutoff: float, ncv: int = 3,
+                noise: float = 0.0, noise_type: str = "wavefunction", orthogonal_to=(),
+                seed: int = 0, precision: str = "fp64", refine: bool = True):
+    """``sweep_`` as a generator that yields ``(energy, discarded)`` after every bond, so a
+    scheduler can interleave bonds of independent solves."""
     if noise_type not in ("wavefunction", "perturbative"):
         raise ValueError(f"noise_type={noise_type!r} is not one of ('wavefunction', 'perturbative')")
     if noise and noise_type == "perturbative":
         raise NotImplementedError("perturbative noise is not part of this build")
     n_sites = len(psi)
-    energy, max_dw = 0.0, 0.0
     for direction in ("right", "left"):
         forward = direction == "right"
         bonds = range(n_sites - 1) if forward else range(n_sites - 2, -1, -1)
         for n in bonds:
-            aa = _pair(psi[n], psi[n + 1])
-            op = env.operator(n, precision)
-            ref = None
-            if precision != "fp64" and refine:
-                ref = env.operator(n, "fp64")
-            energy, aa = lanczos(op, aa, ncv=ncv, refine=ref)
-            if noise:
-                r = SymmetricTensor.random(aa.legs, seed=seed + n, device=aa.device)
-                aa = ops.add(aa, ops.multiply(r, float(noise * ops.norm(aa) / ops.norm(r))))
-                aa = ops.divide(aa, ops.norm(aa))
-            u, s, vh = linalg.svd_truncated(aa, ((0, 1), (2, 3)), max_bond=chi, cutoff=cutoff)
-            norm_s = float(ops.norm(s))
-            max_dw = max(max_dw, 1.0 - (norm_s / float(ops.norm(aa))) ** 2)
-            s = ops.divide(s, norm_s)
-            psi[n + 1] = vh
-            if forward:
-                psi[n] = u
-                psi[n + 1] = ops.einsum("xy,yqr->xqr", s, psi[n + 1])
-            else:
-                psi[n] = ops.einsum("apx,xy->apy", u, s)
-            schmidt[n] = spectrum(s)
-            psi.center = n + 1 if forward else n
-            for e in (env, *orthogonal_to):
-                e.clear_(n, n + 1)
-                if forward:
-                    e.update_(n, to="last")
-                else:
-                    e.update_(n + 1, to="first")
+            yield _bond_step(psi, env, schmidt, n, forward, chi=chi, cutoff=cutoff, ncv=ncv,
+                             noise=noise, seed=seed, precision=precision, refine=refine,
+                             orthogonal_to=orthogonal_to)
+
+
+def sweep_(psi, h, env, schmidt: dict, *, chi: int, cutoff: float, ncv: int = 3,
+           noise: float = 0.0, noise_type: str = "wavefunction", orthogonal_to=(), seed: int = 0,
+           precision: str = "fp64", refine: bool = True):
+    """One left-to-right then right-to-left two-site sweep; ``psi`` and ``env`` mutate.
+    Returns ``(last energy, max discarded weight)`` (reference ``dmrg.py:321-542``).
+
+    ``precision`` selects the H_eff arithmetic (``fp64``, ``fp32``, ``bf16x3``); with a
+    reduced precision and ``refine`` the energy of every bond is the FP64 Rayleigh quotient.
+    """
+    energy, max_dw = 0.0, 0.0
+    for e, dw in sweep_steps(psi, h, env, schmidt, chi=chi, cutoff=cutoff, ncv=ncv, noise=noise,
+                             noise_type=noise_type, orthogonal_to=orthogonal_to, seed=seed,
+                             precision=precision, refine=refine):
+        energy, max_dw = e, max(max_dw, dw)
     return energy, max_dw
 
 
@@ -187,15 +207,13 @@ def _schmidt_change(old: dict, new: dict) -> float:
     return worst
 
 
-def dmrg_(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1e-12,
-          schmidt_tol: float = 1e-8, max_sweeps: int = 40, ncv: int = 3, orthogonal_to=None,
-          seed: int = 0, callback=None, compile=None, precision: str = "fp64",
-          refine: bool = True) -> DMRG_out:
-    """Sweep ``psi`` to the ground state of ``h`` in place (reference ``dmrg.py:620-843``)."""
+def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1e-12,
+               schmidt_tol: float = 1e-8, max_sweeps: int = 40, ncv: int = 3, seed: int = 0,
+               callback=None, compile=None, precision: str = "fp64", refine: bool = True):
+    """``dmrg_`` as a generator: yields ``None`` after every bond and returns the
+    ``DMRG_out`` (``StopIteration.value``) when the loop ends."""
     from tenet.network.env import Env
 
-    if orthogonal_to:
-        raise NotImplementedError("excited-state projection is not part of this build")
     if schedule is not None:
         if chi is not None or cutoff is not None:
             raise ValueError("pass either schedule=[Sweep(...)] or the flat chi=/cutoff= kwargs, not both")
@@ -213,9 +231,12 @@ def dmrg_(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1
     for it in range(1, max_sweeps + 1):
         entry = plan[it - 1] if it <= len(plan) else plan[-1]
         old_e, old_s = energy, dict(schmidt)
-        energy, max_dw = sweep_(psi, h, env, schmidt, chi=entry.chi, cutoff=entry.cutoff, ncv=ncv,
-                                noise=entry.noise, noise_type=entry.noise_type,
-                                seed=seed + 977 * it, precision=precision, refine=refine)
+        max_dw = 0.0
+        for e, dw in sweep_steps(psi, h, env, schmidt, chi=entry.chi, cutoff=entry.cutoff, ncv=ncv,
+                                 noise=entry.noise, noise_type=entry.noise_type,
+                                 seed=seed + 977 * it, precision=precision, refine=refine):
+            energy, max_dw = e, max(max_dw, dw)
+            yield None
         de = abs(old_e - energy)
         ds = _schmidt_change(old_s, schmidt)
         history.append((energy, de, ds, max_dw))
@@ -226,3 +247,20 @@ def dmrg_(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1
         if de < energy_tol and ds < schmidt_tol and it >= len(plan) and entry.noise == 0.0:
             break
     return out
+
+
+def dmrg_(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1e-12,
+          schmidt_tol: float = 1e-8, max_sweeps: int = 40, ncv: int = 3, orthogonal_to=None,
+          seed: int = 0, callback=None, compile=None, precision: str = "fp64",
+          refine: bool = True) -> DMRG_out:
+    """Sweep ``psi`` to the ground state of ``h`` in place (reference ``dmrg.py:620-843``)."""
+    if orthogonal_to:
+        raise NotImplementedError("excited-state projection is not part of this build")
+    gen = dmrg_steps(psi, h, schedule=schedule, chi=chi, cutoff=cutoff, energy_tol=energy_tol,
+                     schmidt_tol=schmidt_tol, max_sweeps=max_sweeps, ncv=ncv, seed=seed,
+                     callback=callback, compile=compile, precision=precision, refine=refine)
+    while True:
+        try:
+            next(gen)
+        except StopIteration as stop:
+            return stop.value

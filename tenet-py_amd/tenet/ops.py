@@ -265,7 +265,9 @@ def _weights(structure, device):
 
 
 def _work(device) -> torch.Tensor:
-    key = (device.type, device.index)
+    """Partials buffer of the two-pass dot, one per (device, stream): reductions issued on
+    different streams must not share it."""
+    key = (device.type, device.index, _native.stream_ptr())
     w = _WORK.get(key)
     if w is None:
         w = _WORK[key] = torch.empty(1024, dtype=torch.float64, device=device)
