@@ -99,8 +99,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-blocks", action="store_true")
     ap.add_argument("--arms", default="fp32,bf16x3,fp64")
-    ap.add_argument("--workload", default="heff2", choices=["heff2", "einsum", "batched"])
-    ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--workload", default="heff2", choices=["heff2", "einsum", "batched", "dmrg", "serve"])
+    ap.add_argument("--count", type=int, default=None)
+    ap.add_argument("--chi", type=int, default=128)
+    ap.add_argument("--rate", type=float, default=2.0)
+    ap.add_argument("--max-sweeps", type=int, default=30)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -110,8 +113,14 @@ def main() -> int:
 
     if args.workload == "einsum":
         return bench_einsum(args)
+    if args.count is None:
+        args.count = 12 if args.workload == "serve" else 256
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "dmrg":
+        return bench_dmrg(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     return bench_heff2(args)
 
 
@@ -320,6 +329,90 @@ def bench_batched(args) -> int:
             "roofline": {"bound": "mfma", "achieved": results[head]["effective_tflops"],
                          "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
                          "frac": results[head]["effective_tflops"] / PEAK_TFLOPS[head], "traffic": None}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+MPSKIT = {32: {64: -13.99731561797256, 128: -13.997315618224851, 256: -13.997315618224897},
+          64: {64: -28.175424792530414, 128: -28.17542485963937, 256: -28.175424859745323}}
+
+
+def bench_dmrg(args) -> int:
+    """One ground-state solve of the U(1) Heisenberg chain; s/sweep and energy against the
+    MPSKit fixture energies the reference's tests hold."""
+    import torch
+
+    from tenet.network import dmrg_
+    from tenet.network.models import heisenberg_mpo, random_chain
+
+    L = args.sites if args.sites != 64 or args.chi in (64, 128, 256) else args.sites
+    res = {}
+    for p in [a for a in args.arms.split(",") if a in ("fp64", "fp32", "bf16x3")]:
+        psi = random_chain(L, 16, 3.0, seed=1, device="cuda")
+        h = heisenberg_mpo(L, device="cuda")
+        times = []
+        last = [time.perf_counter()]
+
+        def cb(out):
+            torch.cuda.synchronize()
+            now = time.perf_counter()
+            times.append(now - last[0])
+            last[0] = now
+            log(f"{p} sweep {out.sweeps}: E={out.energy:.15f} dE={out.denergy:.2e} dS={out.max_dSchmidt:.2e} ({times[-1]:.2f}s)")
+
+        t0 = time.perf_counter()
+        out = dmrg_(psi, h, chi=args.chi, max_sweeps=args.max_sweeps, precision=p, callback=cb)
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t0
+        ref = MPSKIT.get(L, {}).get(args.chi)
+        res[p] = {"energy": out.energy, "sweeps": out.sweeps, "total_s": total,
+                  "s_per_sweep_steady": sorted(times)[len(times) // 2] if times else None,
+                  "first_sweep_s": times[0] if times else None,
+                  "mpskit_energy": ref, "dE_vs_mpskit": (abs(out.energy - ref) if ref else None)}
+    if "fp64" in res:
+        for p in res:
+            res[p]["dE_rel_vs_fp64"] = abs(res[p]["energy"] - res["fp64"]["energy"]) / abs(res["fp64"]["energy"])
+    head = "fp32" if "fp32" in res else next(iter(res))
+    line = {"metric": "DMRG seconds per steady sweep", "value": res[head]["s_per_sweep_steady"],
+            "unit": "s/sweep", "n_gpus": 1, "steps": res[head]["sweeps"], "warmup": 0,
+            "ms_per_step": 1e3 * res[head]["s_per_sweep_steady"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32" if head != "fp64" else "f64",
+            "data": "synthetic", "config": {"workload": f"U(1) Heisenberg OBC L={L} chi={args.chi} DMRG"},
+            "arms": res}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_serve(args) -> int:
+    """Poisson stream of XXZ ground-state solves through the continuous-batching server,
+    against the same stream solved one at a time in FP64."""
+    from tenet.network.serve import DMRGServer, latency_report, poisson_requests
+
+    sizes = tuple(int(x) for x in os.environ.get("TENET_SERVE_SIZES", "32,64").split(","))
+    chis = tuple(int(x) for x in os.environ.get("TENET_SERVE_CHIS", "64,128").split(","))
+    reqs = poisson_requests(args.count, args.rate, sizes=sizes, chis=chis, seed=0,
+                            max_sweeps=args.max_sweeps)
+    out = {}
+    for name, kw in (("continuous_fp32", {"max_active": 8, "precision": "fp32"}),
+                     ("one_at_a_time_fp64", {"max_active": 1, "precision": "fp64"})):
+        t0 = time.perf_counter()
+        res = DMRGServer(**kw).serve(reqs)
+        wall = time.perf_counter() - t0
+        rep = latency_report(res, wall)
+        rep["energies"] = [r.energy for r in res]
+        out[name] = rep
+        log(f"{name}: p50 {rep['p50_s']:.2f}s p99 {rep['p99_s']:.2f}s, {rep['solves_per_hour']:.0f} solves/h")
+    e32, e64 = out["continuous_fp32"]["energies"], out["one_at_a_time_fp64"]["energies"]
+    de = max(abs(a - b) / abs(b) for a, b in zip(e32, e64))
+    head = out["continuous_fp32"]
+    line = {"metric": "serve p50/p99 time-to-solve", "value": head["solves_per_hour"],
+            "unit": "solves/hour", "n_gpus": 1, "steps": args.count, "warmup": 0,
+            "ms_per_step": 1e3 * head["mean_s"], "higher_is_better": True, "scaling": "strong",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": f"Poisson stream of {args.count} XXZ DMRG solves, rate {args.rate}/s, "
+                                   f"L in {sizes}, chi in {chis}, Delta~U[0,2], cutoff 1e-10"},
+            "p50_s": head["p50_s"], "p99_s": head["p99_s"], "max_dE_rel_vs_fp64_serial": de,
+            "arms": {k: {kk: vv for kk, vv in v.items() if kk != "energies"} for k, v in out.items()}}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -121,6 +121,30 @@ int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t* jobs, int6
                  const int64_t* in_slabs, const int64_t* out_slabs, const void* coef,
                  const int64_t* tiles, int64_t ntiles, void* stream);
 
+/* Batched one-sided Jacobi SVD (FP64) of the sector matrices of one blockwise SVD: replaces
+ * the per-coupled-sector ar.do("linalg.svd", ...) of ops/linalg.py:229 (svd), :1436
+ * (select_bond) and :1593 (svd_truncated).  Problem p factors "k vectors of length L"
+ * stored row-major at x + x_off (the rows of A when rows <= cols, else the rows of A^T);
+ * q + q_off holds its k x k orthogonal factor.  Problem table, TN_SVD_PROB_WORDS int64:
+ *   [0] k [1] L [2] x_off [3] q_off [4] nb (even block count, blocks of w vectors)
+ *   [5] cols (0: x holds A, 1: x holds A^T) [6] u_off [7] s_off [8] vh_off
+ *   [9] inner sweeps per block-pair visit
+ * tn_svd_jacobi_round runs round `round` of the block tournament: workgroup i takes block
+ * pair slot wgs[2i+1] of problem wgs[2i]; span = the largest L + k of the launch (LDS is
+ * 2 w span doubles); first != 0 initialises Q = I on load (round 0 of the first sweep);
+ * rotations[i] receives the rotation count of its last inner sweep.  A pair rotates when
+ * |<x_a,x_b>| > tol * |x_a| |x_b|.
+ * tn_svd_jacobi_finish writes, per problem, S (k, descending), U (rows x k, row-major) and
+ * Vh (k x cols, row-major) of the thin SVD. */
+#define TN_SVD_PROB_WORDS 10
+#define TN_SVD_MAX_W 32
+#define TN_SVD_MAX_K 4096
+int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int64_t nwgs, int64_t span,
+                        double* x, double* q, int32_t* rotations, int w, int round, int first,
+                        double tol, void* stream);
+int tn_svd_jacobi_finish(const int64_t* probs, int64_t nprobs, const double* x, const double* q,
+                         double* U, double* S, double* Vh, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

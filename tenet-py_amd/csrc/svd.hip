@@ -1,0 +1,247 @@
+// Batched one-sided Jacobi SVD in FP64 for the truncating blockwise SVD.
+//
+// Every sector matrix of one SVD call is a problem "k vectors of length L" stored row-major
+// (rows of A when rows <= cols, else rows of A^T).  One-sided (Hestenes) Jacobi rotates
+// pairs of vectors until all are mutually orthogonal; the same rotations applied to the
+// rows of Q (k x k, starts as I) record the orthogonal factor.  At convergence
+// X = Q A_rows with orthogonal rows, so A_rows = Q^T diag(sigma) W, W's rows orthonormal.
+//
+// Vectors are grouped in blocks of w.  One launch is one round of a circle-method
+// tournament over the blocks of every problem: each workgroup takes one block pair, stages
+// its 2w vectors and their Q rows in LDS, runs inner tournaments over the 2w vectors
+// (every pair once per inner sweep; the w pairs of a round side by side, one 32-lane
+// group each), and writes both back.
+// A problem whose vectors fit in one block pair converges inside a single launch.
+// Each workgroup reports the rotations of its last inner sweep, so the host stops when a
+// whole sweep reports none.
+
+#include "tn_common.hpp"
+
+namespace {
+
+__device__ inline double wave_sum(double v) {
+  // butterfly: every lane ends with the same value (IEEE addition commutes exactly)
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline double half_sum(double v) {
+  // butterfly within each 32-lane half of the wave
+  for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+  return v;
+}
+
+// circle method over n (even) players: round r in [0, n-1), pair p in [0, n/2)
+__device__ inline void pair_of(int n, int r, int p, int& a, int& b) {
+  a = p == 0 ? 0 : ((p - 1 + r) % (n - 1)) + 1;
+  b = ((n - 2 - p + r) % (n - 1)) + 1;
+}
+
+__global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* __restrict__ wgs,
+                             double* __restrict__ x, double* __restrict__ q,
+                             int32_t* __restrict__ rot_out, int w, int round, int first,
+                             double tol) {
+  extern __shared__ double lds[];
+  __shared__ int rot_s;
+  const int64_t* pw = probs + TN_SVD_PROB_WORDS * wgs[2 * blockIdx.x];
+  const int slot = wgs[2 * blockIdx.x + 1];
+  const int k = static_cast<int>(pw[0]);
+  const int L = static_cast<int>(pw[1]);
+  const int64_t xo = pw[2], qo = pw[3];
+  const int nb = static_cast<int>(pw[4]);
+  const int inner_sweeps = static_cast<int>(pw[9]);
+  if (round >= nb - 1) {
+    if (threadIdx.x == 0) rot_out[blockIdx.x] = 0;
+    return;
+  }
+  int bi, bj;
+  pair_of(nb, round, slot, bi, bj);
+  const int n2 = 2 * w;
+  double* xs = lds;                                  // n2 x L
+  double* qs = lds + static_cast<size_t>(n2) * L;    // n2 x k
+  for (int t = 0; t < n2; ++t) {
+    const int g = t < w ? bi * w + t : bj * w + (t - w);
+    if (g < k) {
+      const double* xr = x + xo + static_cast<int64_t>(g) * L;
+      const double* qr = q + qo + static_cast<int64_t>(g) * k;
+      for (int i = threadIdx.x; i < L; i += blockDim.x) xs[t * L + i] = xr[i];
+      for (int i = threadIdx.x; i < k; i += blockDim.x)
+        qs[t * k + i] = first ? (i == g ? 1.0 : 0.0) : qr[i];
+    } else {
+      for (int i = threadIdx.x; i < L; i += blockDim.x) xs[t * L + i] = 0.0;
+      for (int i = threadIdx.x; i < k; i += blockDim.x) qs[t * k + i] = 0.0;
+    }
+  }
+  // one pair per 32-lane group: a round's w pairs run side by side
+  const int gl = threadIdx.x & 31;
+  const int grp = threadIdx.x >> 5;
+  const int ngrp = blockDim.x >> 5;
+  const double tol2 = tol * tol;
+  int last = 0;
+  for (int it = 0; it < inner_sweeps; ++it) {
+    if (threadIdx.x == 0) rot_s = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int r = 0; r < n2 - 1; ++r) {
+      for (int p = grp; p < w; p += ngrp) {
+        int a, b;
+        pair_of(n2, r, p, a, b);
+        double* xa = xs + a * L;
+        double* xb = xs + b * L;
+        double al = 0.0, be = 0.0, ga = 0.0;
+        for (int i = gl; i < L; i += 32) {
+          const double u = xa[i], v = xb[i];
+          al = fma(u, u, al);
+          be = fma(v, v, be);
+          ga = fma(u, v, ga);
+        }
+        al = half_sum(al);
+        be = half_sum(be);
+        ga = half_sum(ga);
+        // uniform within the group: every lane of it holds the same sums
+        if (al > 0.0 && be > 0.0 && ga * ga > tol2 * al * be) {
+          // t = tan(theta), |theta| <= pi/4, zeroes <x_a, x_b>; applied as
+          // x + ((c - 1) x - s y) with c - 1 = -t^2 / (1 + t^2 + r), r = sqrt(1 + t^2),
+          // formed directly so that small-angle rotations (c rounding to exactly 1) do
+          // not grow every norm by t^2 / 2 a step
+          const double d = be - al;
+          const double g2 = 2.0 * ga;
+          const double hyp = sqrt(fma(d, d, g2 * g2));
+          const double t = g2 / (d + copysign(hyp, d));
+          const double tt = t * t;
+          const double rr = sqrt(1.0 + tt);
+          const double den = 1.0 + tt + rr;
+          const double qd = 1.0 / (rr * den);
+          const double s = t * den * qd;
+          const double cm1 = -tt * rr * qd;
+          for (int i = gl; i < L; i += 32) {
+            const double u = xa[i], v = xb[i];
+            xa[i] = u + fma(cm1, u, -s * v);
+            xb[i] = v + fma(cm1, v, s * u);
+          }
+          double* qa = qs + a * k;
+          double* qb = qs + b * k;
+          for (int i = gl; i < k; i += 32) {
+            const double u = qa[i], v = qb[i];
+            qa[i] = u + fma(cm1, u, -s * v);
+            qb[i] = v + fma(cm1, v, s * u);
+          }
+          ++mine;
+        }
+      }
+      __syncthreads();
+    }
+    if (gl == 0 && mine) atomicAdd(&rot_s, mine);
+    __syncthreads();
+    last = rot_s;
+    __syncthreads();
+    if (last == 0) break;
+  }
+  for (int t = 0; t < n2; ++t) {
+    const int g = t < w ? bi * w + t : bj * w + (t - w);
+    if (g >= k) continue;
+    double* xr = x + xo + static_cast<int64_t>(g) * L;
+    double* qr = q + qo + static_cast<int64_t>(g) * k;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) xr[i] = xs[t * L + i];
+    for (int i = threadIdx.x; i < k; i += blockDim.x) qr[i] = qs[t * k + i];
+  }
+  if (threadIdx.x == 0) rot_out[blockIdx.x] = last;
+}
+
+// One workgroup per problem: singular values = row norms, descending order by rank
+// counting, normalized rows and Q rows scattered into U / S / Vh.
+__global__ void jacobi_finish(const int64_t* __restrict__ probs, const double* __restrict__ x,
+                              const double* __restrict__ q, double* __restrict__ U,
+                              double* __restrict__ S, double* __restrict__ Vh) {
+  extern __shared__ double sig[];
+  int* rank = reinterpret_cast<int*>(sig + TN_SVD_MAX_K);
+  const int64_t* pw = probs + TN_SVD_PROB_WORDS * blockIdx.x;
+  const int k = static_cast<int>(pw[0]);
+  const int L = static_cast<int>(pw[1]);
+  const int64_t xo = pw[2], qo = pw[3];
+  const int cols = static_cast<int>(pw[5]);
+  const int64_t uo = pw[6], so = pw[7], vo = pw[8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int i = wave; i < k; i += nw) {
+    const double* xr = x + xo + static_cast<int64_t>(i) * L;
+    double acc = 0.0;
+    for (int j = lane; j < L; j += 64) acc = fma(xr[j], xr[j], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) sig[i] = sqrt(acc);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    const double si = sig[i];
+    int rk = 0;
+    for (int j = 0; j < k; ++j) rk += (sig[j] > si) || (sig[j] == si && j < i);
+    rank[i] = rk;
+    S[so + rk] = si;
+  }
+  __syncthreads();
+  for (int i = 0; i < k; ++i) {
+    const int rk = rank[i];
+    const double si = sig[i];
+    const double inv = si > 0.0 ? 1.0 / si : 0.0;
+    const double* xr = x + xo + static_cast<int64_t>(i) * L;
+    const double* qr = q + qo + static_cast<int64_t>(i) * k;
+    if (!cols) {
+      // A (k x L) = Q^T diag(sigma) W: U[:, rk] = Q row i, Vh[rk, :] = W row i
+      for (int r = threadIdx.x; r < k; r += blockDim.x) U[uo + static_cast<int64_t>(r) * k + rk] = qr[r];
+      for (int j = threadIdx.x; j < L; j += blockDim.x) Vh[vo + static_cast<int64_t>(rk) * L + j] = xr[j] * inv;
+    } else {
+      // A^T (k x L) = Q^T diag(sigma) W: U[:, rk] = W row i, Vh[rk, :] = Q row i
+      for (int j = threadIdx.x; j < L; j += blockDim.x) U[uo + static_cast<int64_t>(j) * k + rk] = xr[j] * inv;
+      for (int r = threadIdx.x; r < k; r += blockDim.x) Vh[vo + static_cast<int64_t>(rk) * k + r] = qr[r];
+    }
+  }
+}
+
+// Dynamic LDS above the default needs the per-function opt-in; the request must stay
+// within the device's per-workgroup limit (less what the kernel declares statically).
+int allow_lds(const void* fn, size_t bytes, const char* who) {
+  int dev = 0, limit = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return tn::fail(std::string(who) + ": cannot query the LDS limit");
+  if (bytes + 256 > static_cast<size_t>(limit))
+    return tn::fail(std::string(who) + ": " + std::to_string(bytes) + " bytes of LDS requested, device allows " +
+                    std::to_string(limit));
+  if (bytes > 65536 &&
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)) != hipSuccess)
+    return tn::fail(std::string(who) + ": cannot raise the LDS limit to " + std::to_string(bytes));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int64_t nwgs,
+                                   int64_t span, double* x, double* q, int32_t* rotations, int w,
+                                   int round, int first, double tol, void* stream) {
+  if (nwgs <= 0) return 0;
+  if (w < 1 || w > TN_SVD_MAX_W) return tn::fail("tn_svd_jacobi_round: block width out of range");
+  if (nwgs > 0x7fffffff) return tn::fail("tn_svd_jacobi_round: too many workgroups");
+  // LDS per workgroup: 2w vectors of length L plus their 2w Q rows of length k, where
+  // span is the largest L + k of the launch
+  const size_t bytes = static_cast<size_t>(2 * w) * static_cast<size_t>(span) * sizeof(double);
+  if (span <= 0) return tn::fail("tn_svd_jacobi_round: empty launch span");
+  if (int rc = allow_lds(reinterpret_cast<const void*>(jacobi_round), bytes, "tn_svd_jacobi_round"))
+    return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int threads = w <= 2 ? 64 : 32 * w;
+  hipLaunchKernelGGL(jacobi_round, dim3(static_cast<unsigned>(nwgs)), dim3(threads), bytes, s, probs,
+                     wgs, x, q, rotations, w, round, first, tol);
+  return tn::check_launch("jacobi_round");
+}
+
+extern "C" int tn_svd_jacobi_finish(const int64_t* probs, int64_t nprobs, const double* x,
+                                    const double* q, double* U, double* S, double* Vh,
+                                    void* stream) {
+  if (nprobs <= 0) return 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t bytes = TN_SVD_MAX_K * (sizeof(double) + sizeof(int));
+  if (int rc = allow_lds(reinterpret_cast<const void*>(jacobi_finish), bytes, "tn_svd_jacobi_finish"))
+    return rc;
+  hipLaunchKernelGGL(jacobi_finish, dim3(static_cast<unsigned>(nprobs)), dim3(1024), bytes, s, probs,
+                     x, q, U, S, Vh);
+  return tn::check_launch("jacobi_finish");
+}

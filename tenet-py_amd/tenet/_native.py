@@ -22,6 +22,9 @@ MAX_RANK = 6
 PROB_WORDS = 8
 SEG_WORDS = 6
 MPO_JOB_WORDS = 8
+SVD_PROB_WORDS = 10
+SVD_MAX_W = 32
+SVD_MAX_K = 4096
 
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -39,6 +42,8 @@ _SIGNATURES = {
     "tn_scale": (_i32, [_i32, _i64, _d, _p, _i32, _p, _p]),
     "tn_cast": (_i32, [_i32, _p, _i32, _p, _i64, _p]),
     "tn_mpo_apply": (_i32, [_i32, _p, _p, _p, _i64, _p, _p, _p, _p, _i64, _p]),
+    "tn_svd_jacobi_round": (_i32, [_p, _p, _i64, _i64, _p, _p, _p, _i32, _i32, _i32, _d, _p]),
+    "tn_svd_jacobi_finish": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p]),
 }
 
 _lib = None
@@ -82,9 +87,11 @@ def check(rc: int) -> None:
 
 
 def stream_ptr() -> int:
+    """The raw ``hipStream_t`` of torch's current stream on the current device (the
+    direct query; ``torch.cuda.current_stream()`` builds a Python Stream object per call)."""
     import torch
 
-    return torch.cuda.current_stream().cuda_stream
+    return torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice())
 
 
 def loaded() -> bool:

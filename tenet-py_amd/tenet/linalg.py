@@ -2,8 +2,10 @@
 
 Lowering (reference ``ops/linalg.py:72-90``): repartition to ``(left | right)``, then one
 ``tn_recouple`` launch gathers every block into one dense matrix per coupled sector, all
-packed in a single device buffer.  Each sector matrix is factorized in FP64 on the GPU
-(rocSOLVER through torch-ROCm), the truncation decision runs on the host over the O(D)
+packed in a single device buffer.  Every sector matrix is factorized in FP64 on the GPU by
+one batched one-sided Jacobi SVD (``tn_svd_jacobi_round``/``_finish``: one launch per
+block-tournament round for all sectors together, converged when a whole sweep rotates
+nothing), the truncation decision runs on the host over the O(D)
 singular values with the reference's exact keep rule (global spectrum sorted by
 ``(-sigma, sector, index)``, cutoff modes, greedy walk under the dense budget
 ``sum qdim(c)``; reference ``linalg.py:1079-1324``), and one more ``tn_recouple`` launch
@@ -81,17 +83,25 @@ def _matrix_layout(s: TensorStructure) -> _MatrixLayout:
     return _MatrixLayout(s)
 
 
-def to_matrices(t: SymmetricTensor) -> dict:
-    """``{c: B_c}`` device matrices (views into one packed buffer), rows OUT, columns IN
-    (reference ``map_view.py:238-279``)."""
+def _packed(t: SymmetricTensor):
     require_device(t)
     ml = _matrix_layout(t.structure)
     buf = torch.empty(ml.size, dtype=t.dtype, device=t.device)
     ml.to_prog.run(t.data, buf)
+    return buf, ml
+
+
+def _views(buf, ml) -> dict:
     return {
         c: buf[ml.offsets[c] : ml.offsets[c] + ml.shapes[c][0] * ml.shapes[c][1]].view(ml.shapes[c])
         for c in ml.sectors
     }
+
+
+def to_matrices(t: SymmetricTensor) -> dict:
+    """``{c: B_c}`` device matrices (views into one packed buffer), rows OUT, columns IN
+    (reference ``map_view.py:238-279``)."""
+    return _views(*_packed(t))
 
 
 def from_matrices(structure: TensorStructure, mats: dict) -> SymmetricTensor:
@@ -112,11 +122,13 @@ def from_matrices(structure: TensorStructure, mats: dict) -> SymmetricTensor:
 
 
 def _lower(t: SymmetricTensor, axes):
+    """``(repartitioned tensor, {c: B_c}, (packed buffer, layout))``."""
     if axes is None:
         axes = (t.structure.out_axes, t.structure.in_axes)
     left, right = axes
     m = repartition(t, left, right)
-    return m, to_matrices(m)
+    buf, ml = _packed(m)
+    return m, _views(buf, ml), (buf, ml)
 
 
 # ----------------------------------------------------------------------------- keep rule
@@ -224,20 +236,153 @@ def _decide(spectrum, provider, max_bond, cutoff, cutoff_mode, renorm, caller) -
 # ----------------------------------------------------------------------------- factorizations
 
 
-def _svd_parts(mats: dict) -> dict:
-    """FP64 SVD of every sector matrix on the device; equal shapes are batched."""
+_SVD_LDS_BYTES = None     # per-workgroup LDS the block pair may use (device limit less a margin)
+_SVD_MAX_SWEEPS = 60
+_SVD_INNER_SWEEPS = 1     # inner sweeps per block-pair visit when a problem spans several blocks
+_SVD_SINGLE_SWEEPS = 64   # cap for a problem that fits in one block pair (converges in-launch)
+_SVD_LAST_SWEEPS = 0      # sweeps the last call needed (diagnostics)
+
+
+def _lds_budget() -> int:
+    global _SVD_LDS_BYTES
+    if _SVD_LDS_BYTES is None:
+        props = torch.cuda.get_device_properties(torch.cuda.current_device())
+        limit = int(getattr(props, "shared_memory_per_block_optin", 0) or props.shared_memory_per_block)
+        _SVD_LDS_BYTES = limit - 1024
+    return _SVD_LDS_BYTES
+
+
+def _svd_width(span: int, k: int) -> int:
+    """Vectors per block: the largest power of two <= 32 whose block pair (2w vectors of
+    length L and their Q rows of length k, span = L + k doubles each) fits in LDS, and no
+    wider than the smallest pair that holds all k vectors (an inner sweep of a block pair
+    is 2w - 1 dependent rounds, so a problem of 10 vectors runs 15 rounds, not 63)."""
+    budget = _lds_budget()
+    w = 32
+    while w > 1 and (2 * w * span * 8 > budget or w >= k):
+        w //= 2
+    if 2 * w * span * 8 > budget:
+        raise ValueError(f"svd: a sector matrix with rows + cols = {span} exceeds the Jacobi "
+                         f"kernel's LDS block pair ({budget // 16} doubles)")
+    return w
+
+
+class _JacobiPlan:
+    """Problem tables of the batched Jacobi SVD for one packed matrix layout."""
+
+    def __init__(self, ml: _MatrixLayout) -> None:
+        from tenet import _native
+        from tenet.engine import _DeviceCache
+
+        regions, rows, widths = [], [], []
+        self.slices = {}
+        xo = qo = uo = so = vo = 0
+        for c in ml.sectors:
+            m, n = ml.shapes[c]
+            if m == 0 or n == 0:
+                self.slices[c] = None
+                continue
+            cols = m > n
+            k, L = (n, m) if cols else (m, n)
+            if k > _native.SVD_MAX_K:
+                raise ValueError(f"svd: sector {c!r} has {k} singular values, more than the "
+                                 f"kernel's {_native.SVD_MAX_K}")
+            src = ml.offsets[c]
+            if cols:  # x holds A^T: element (j, i) <- A[i, j]
+                regions.append((xo, (n, m), (m, 1), (1, n), [(src, 1.0)]))
+            else:
+                regions.append((xo, (m, n), (n, 1), (n, 1), [(src, 1.0)]))
+            w = _svd_width(L + k, k)
+            nb = max(2, -(-k // w))
+            nb += nb % 2
+            inner = _SVD_SINGLE_SWEEPS if nb == 2 else _SVD_INNER_SWEEPS
+            rows.append([k, L, xo, qo, nb, int(cols), uo, so, vo, inner])
+            widths.append(w)
+            self.slices[c] = (uo, m, k, so, vo, n)
+            xo += k * L
+            qo += k * k
+            uo += m * k
+            so += k
+            vo += k * n
+        self.x_size, self.q_size, self.u_size, self.s_size, self.v_size = xo, qo, uo, so, vo
+        self.gather = RecoupleProgram(regions, ml.size, xo) if regions else None
+        self.nprobs = len(rows)
+        self.probs = _DeviceCache(probs=np.asarray(rows, dtype=np.int64).reshape(-1))
+        self.groups = []
+        for w in sorted(set(widths), reverse=True):
+            sub = [r for r, ww in zip(rows, widths) if ww == w]
+            wgs = [(p, slot) for p, r in enumerate(sub) for slot in range(r[4] // 2)]
+            span = max(r[0] + r[1] for r in sub)
+            # rotate while |cos(x_a, x_b)| > tol: 8 eps sqrt(L) sits above the rounding
+            # noise of a length-L dot product, so converged pairs stop rotating on noise
+            tol = 8 * 2.220446049250313e-16 * max(1.0, max(r[1] for r in sub) ** 0.5)
+            self.groups.append(dict(
+                w=w, span=span, tol=tol, rounds=max(r[4] for r in sub) - 1, nwgs=len(wgs),
+                tables=_DeviceCache(probs=np.asarray(sub, dtype=np.int64).reshape(-1),
+                                    wgs=np.asarray(wgs, dtype=np.int32).reshape(-1))))
+
+
+def _jacobi_plan(ml: _MatrixLayout) -> _JacobiPlan:
+    plan = getattr(ml, "_jacobi", None)
+    if plan is None:
+        plan = ml._jacobi = _JacobiPlan(ml)
+    return plan
+
+
+def _svd_parts(packed) -> dict:
+    """FP64 thin SVD of every packed sector matrix with the batched Jacobi kernels:
+    ``{c: (U, S, Vh)}``, singular values descending, as device views."""
+    from tenet import _native
+
+    buf, ml = packed
+    if buf.dtype != torch.float64:
+        raise TypeError("svd: the Jacobi SVD runs in float64; cast the tensor first")
+    plan = _jacobi_plan(ml)
+    dev = buf.device
+    lib = _native.lib()
+    stream = _native.stream_ptr()
+    x = torch.empty(plan.x_size, dtype=torch.float64, device=dev)
+    q = torch.empty(plan.q_size, dtype=torch.float64, device=dev)
+    if plan.gather is not None:
+        plan.gather.run(buf, x)
+    rots = []
+    for g in plan.groups:
+        g["tb"] = g["tables"].get(dev)
+        rots.append(torch.zeros(max(g["rounds"], 1) * g["nwgs"], dtype=torch.int32, device=dev))
+    live = list(range(len(plan.groups)))
+    global _SVD_LAST_SWEEPS
+    for sweep in range(_SVD_MAX_SWEEPS):
+        _SVD_LAST_SWEEPS = sweep + 1
+        for gi in live:
+            g, rot = plan.groups[gi], rots[gi]
+            for r in range(g["rounds"]):
+                _native.check(lib.tn_svd_jacobi_round(
+                    g["tb"]["probs"].data_ptr(), g["tb"]["wgs"].data_ptr(), g["nwgs"], g["span"],
+                    x.data_ptr(), q.data_ptr(), rot[r * g["nwgs"]:].data_ptr(), g["w"], r,
+                    int(sweep == 0 and r == 0), g["tol"], stream))
+        counts = torch.stack([rots[gi].sum() for gi in live]).cpu().tolist()
+        live = [gi for gi, n in zip(live, counts) if n]
+        if not live:
+            break
+    else:
+        raise RuntimeError(f"svd: the Jacobi sweeps did not converge in {_SVD_MAX_SWEEPS} sweeps")
+    U = torch.empty(plan.u_size, dtype=torch.float64, device=dev)
+    S = torch.empty(plan.s_size, dtype=torch.float64, device=dev)
+    Vh = torch.empty(plan.v_size, dtype=torch.float64, device=dev)
+    if plan.nprobs:
+        _native.check(lib.tn_svd_jacobi_finish(
+            plan.probs.get(dev)["probs"].data_ptr(), plan.nprobs, x.data_ptr(), q.data_ptr(),
+            U.data_ptr(), S.data_ptr(), Vh.data_ptr(), stream))
     parts = {}
-    by_shape: dict = {}
-    for c, m in mats.items():
-        by_shape.setdefault(tuple(m.shape), []).append(c)
-    for shape, cs in by_shape.items():
-        if len(cs) == 1:
-            u, s, vh = torch.linalg.svd(mats[cs[0]], full_matrices=False)
-            parts[cs[0]] = (u, s, vh)
-        else:
-            u, s, vh = torch.linalg.svd(torch.stack([mats[c] for c in cs]), full_matrices=False)
-            for k, c in enumerate(cs):
-                parts[c] = (u[k], s[k], vh[k])
+    for c in ml.sectors:
+        sl = plan.slices[c]
+        m, n = ml.shapes[c]
+        if sl is None:
+            e = torch.empty(0, dtype=torch.float64, device=dev)
+            parts[c] = (e.view(m, 0), e, e.view(0, n))
+            continue
+        uo, m, k, so, vo, n = sl
+        parts[c] = (U[uo : uo + m * k].view(m, k), S[so : so + k], Vh[vo : vo + k * n].view(k, n))
     return parts
 
 
@@ -256,8 +401,8 @@ def _factors(m: SymmetricTensor, parts: dict, keep: dict, scale: float = 1.0):
 def svd(t: SymmetricTensor, axes=None, *, bond: GradedSpace | None = None):
     """Compact SVD ``T = U S Vh`` per coupled sector, or its projection onto a
     pre-decided ``bond`` (reference ``linalg.py:122-245``)."""
-    m, mats = _lower(t, axes)
-    parts = _svd_parts(mats)
+    m, mats, packed = _lower(t, axes)
+    parts = _svd_parts(packed)
     full = {c: min(mats[c].shape) for c in mats}
     if bond is None:
         keep = full
@@ -277,8 +422,8 @@ def select_bond(t: SymmetricTensor, axes=None, *, max_bond=None, cutoff=None, cu
                 renorm=False) -> BondSelection:
     """The truncation decision alone (reference ``linalg.py:1327-1438``)."""
     _validate(max_bond, cutoff, cutoff_mode, renorm, "select_bond")
-    m, mats = _lower(t, axes)
-    parts = _svd_parts(mats)
+    m, _, packed = _lower(t, axes)
+    parts = _svd_parts(packed)
     return _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, "select_bond")
 
 
@@ -303,8 +448,8 @@ def svd_truncated(t: SymmetricTensor, axes=None, *, max_bond=None, cutoff=None,
     ``(bond OUT, bond IN)``, ``Vh`` on ``(bond OUT, *right legs)``.
     """
     _validate(max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
-    m, mats = _lower(t, axes)
-    parts = _svd_parts(mats)
+    m, _, packed = _lower(t, axes)
+    parts = _svd_parts(packed)
     sel = _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
     return _factors(m, parts, dict(sel.bond.sectors), sel.scale)
 
@@ -313,8 +458,8 @@ def svd_truncated_with_selection(t, axes=None, *, max_bond=None, cutoff=None, cu
                                  renorm=False):
     """``svd_truncated`` that also returns its ``BondSelection``."""
     _validate(max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
-    m, mats = _lower(t, axes)
-    parts = _svd_parts(mats)
+    m, _, packed = _lower(t, axes)
+    parts = _svd_parts(packed)
     sel = _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
     return (*_factors(m, parts, dict(sel.bond.sectors), sel.scale), sel)
 
@@ -322,7 +467,7 @@ def svd_truncated_with_selection(t, axes=None, *, max_bond=None, cutoff=None, cu
 def lq(t: SymmetricTensor, axes=None):
     """``T = L Q`` per coupled sector with ``Q`` row-orthonormal, bond ``min(rows, cols)``
     (reference ``linalg.py:549-620``: QR of each sector matrix's transpose)."""
-    m, mats = _lower(t, axes)
+    m, mats, _ = _lower(t, axes)
     provider = m.provider
     keep = {c: min(b.shape) for c, b in mats.items()}
     bond = GradedSpace.new(provider, keep)

@@ -119,12 +119,12 @@ class BatchedHeff2:
             np.cumsum(sizes, out=o[1:])
             return o
         self.x_off = offsets([s.size for s in self.structures])
-        fr_off = offsets([op._statics[0][0].structure.size for op in self.ops])
-        fl_off = offsets([op._fl_w.structure.size for op in self.ops])
+        fr_off = offsets([op._fr.structure.size for op in self.ops])
+        fl_off = offsets([op._fl.structure.size for op in self.ops])
         t1_off = offsets([ch.t1_size for ch in chains])
         t3_off = offsets([ch.t3_size for ch in chains])
-        self.fr = torch.cat([op._statics[0][0].data for op in self.ops]).to(dev)
-        self.fl = torch.cat([op._fl_w.data for op in self.ops]).to(dev)
+        self.fr = torch.cat([op._fr.data for op in self.ops]).to(dev)
+        self.fl = torch.cat([op._fl.data for op in self.ops]).to(dev)
         self.t1_size, self.t3_size = int(t1_off[-1]), int(t3_off[-1])
         self.x_size = int(self.x_off[-1])
         p1, p4 = [], []

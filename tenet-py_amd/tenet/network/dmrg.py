@@ -209,9 +209,17 @@ def _schmidt_change(old: dict, new: dict) -> float:
 
 def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1e-12,
                schmidt_tol: float = 1e-8, max_sweeps: int = 40, ncv: int = 3, seed: int = 0,
-               callback=None, compile=None, precision: str = "fp64", refine: bool = True):
+               callback=None, compile=None, precision: str = "fp64", refine: bool = True,
+               polish_tol: float | None = None):
     """``dmrg_`` as a generator: yields ``None`` after every bond and returns the
-    ``DMRG_out`` (``StopIteration.value``) when the loop ends."""
+    ``DMRG_out`` (``StopIteration.value``) when the loop ends.
+
+    With a reduced ``precision`` the Schmidt values carry the operator's rounding
+    (|dSchmidt| ~ 1e-6 for FP32), so the Schmidt criterion cannot be met by reduced
+    sweeps alone: once a reduced sweep changes the energy by less than ``polish_tol``
+    (default ``1000 * energy_tol``) the remaining sweeps run the FP64 operator."""
+    if polish_tol is None:
+        polish_tol = 1e3 * energy_tol
     from tenet.network.env import Env
 
     if schedule is not None:
@@ -238,6 +246,8 @@ def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: floa
             energy, max_dw = e, max(max_dw, dw)
             yield None
         de = abs(old_e - energy)
+        if precision != "fp64" and de < polish_tol:
+            precision = "fp64"
         ds = _schmidt_change(old_s, schmidt)
         history.append((energy, de, ds, max_dw))
         realized.append(entry)
@@ -252,13 +262,14 @@ def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: floa
 def dmrg_(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1e-12,
           schmidt_tol: float = 1e-8, max_sweeps: int = 40, ncv: int = 3, orthogonal_to=None,
           seed: int = 0, callback=None, compile=None, precision: str = "fp64",
-          refine: bool = True) -> DMRG_out:
+          refine: bool = True, polish_tol: float | None = None) -> DMRG_out:
     """Sweep ``psi`` to the ground state of ``h`` in place (reference ``dmrg.py:620-843``)."""
     if orthogonal_to:
         raise NotImplementedError("excited-state projection is not part of this build")
     gen = dmrg_steps(psi, h, schedule=schedule, chi=chi, cutoff=cutoff, energy_tol=energy_tol,
                      schmidt_tol=schmidt_tol, max_sweeps=max_sweeps, ncv=ncv, seed=seed,
-                     callback=callback, compile=compile, precision=precision, refine=refine)
+                     callback=callback, compile=compile, precision=precision, refine=refine,
+                     polish_tol=polish_tol)
     while True:
         try:
             next(gen)

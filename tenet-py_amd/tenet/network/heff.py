@@ -13,6 +13,8 @@ fixed sequence on the current stream, so the whole matvec can be captured as a h
 
 from __future__ import annotations
 
+from collections import OrderedDict
+
 import numpy as np
 import torch
 
@@ -49,7 +51,8 @@ def _einsum_axes(t0: str, t1: str, out: str):
 
 
 class _Chain:
-    """The compiled launch sequence for one input structure."""
+    """The compiled launch sequence for one input structure.  ``statics`` holds the
+    ``(structure, term)`` of each step's static operand; their buffers come at run time."""
 
     def __init__(self, s_in: TensorStructure, statics) -> None:
         self.steps = []
@@ -67,23 +70,23 @@ class _Chain:
                     rep = recouple_program(s, plan.new_structure, tuple(plan.perm), tuple(plan.terms))
                 s = plan.new_structure
             axes, out_axes = _einsum_axes(ta, stat[1], out)
-            ex = contraction_exec(s, stat[0].structure, axes, out_axes)
+            ex = contraction_exec(s, stat[0], axes, out_axes)
             self.flops += ex.flops
-            self.steps.append((rep, s, ex, stat[0]))
+            self.steps.append((rep, s, ex))
             s = ex.out_structure
         if s != s_in:
             raise ValueError("heff2: the output structure differs from the input's")
         self.s_out = s
 
-    def run(self, x: torch.Tensor, precision: str, events: list | None = None) -> torch.Tensor:
+    def run(self, x: torch.Tensor, datas, precision: str, events: dict | None = None) -> torch.Tensor:
         cur = x
-        for k, (rep, s, ex, stat) in enumerate(self.steps):
+        for k, ((rep, s, ex), data) in enumerate(zip(self.steps, datas)):
             if rep is not None:
                 nxt = torch.empty(s.size, dtype=cur.dtype, device=cur.device)
                 rep.run(cur, nxt)
                 cur = nxt
             ev = None if events is None else events.setdefault(k, [])
-            cur = ex.run(cur, stat.data, precision, events=ev)
+            cur = ex.run(cur, data, precision, events=ev)
         return cur
 
 
@@ -104,6 +107,44 @@ def _dense(t: SymmetricTensor):
     return out, [o for o, _ in offs]
 
 
+_W_HOST: OrderedDict = OrderedDict()
+
+
+def _w_host(w: SymmetricTensor):
+    """Host dense copy of an MPO site tensor plus a content key, kept per buffer (an MPO's
+    site tensors are reused by every bond of every sweep; one readback each)."""
+    k = id(w.data)
+    hit = _W_HOST.get(k)
+    if hit is not None and hit[0].data is w.data:
+        _W_HOST.move_to_end(k)
+        return hit[1]
+    dense, offs = _dense(w)
+    val = (dense, offs, (w.structure, dense.tobytes()))
+    _W_HOST[k] = (w, val)  # holding w keeps its buffer alive, so the id stays unique
+    if len(_W_HOST) > 256:
+        _W_HOST.popitem(last=False)
+    return val
+
+
+_CHAINS: OrderedDict = OrderedDict()
+_CHAINS_MAX = 1024
+
+
+def _cached_chain(key, build):
+    """Compiled chains are pure functions of structures (and the MPO's values), so every
+    operator with the same key -- the same bond in a later sweep, the FP64 refinement
+    operator beside the FP32 one, another solve of the same model -- shares one."""
+    c = _CHAINS.get(key)
+    if c is None:
+        c = build()
+        _CHAINS[key] = c
+        if len(_CHAINS) > _CHAINS_MAX:
+            _CHAINS.popitem(last=False)
+    else:
+        _CHAINS.move_to_end(key)
+    return c
+
+
 class _FusedChain:
     """Abelian fast path: three launches, no permutation pass.
 
@@ -120,15 +161,16 @@ class _FusedChain:
     path's four contractions are exactly these three steps on the dense-index level.
     """
 
-    def __init__(self, s_in: TensorStructure, fr, fl, w_left, w_right, fr_term) -> None:
+    def __init__(self, s_in: TensorStructure, fr_s: TensorStructure, fl_s: TensorStructure,
+                 w_left, w_right, fr_term) -> None:
         from tenet.engine import GemmProgram, _DeviceCache
 
         axes, out_axes = _einsum_axes("apqr", fr_term, "apqys")
-        self.ex1 = contraction_exec(s_in, fr.structure, axes, out_axes)
+        self.ex1 = contraction_exec(s_in, fr_s, axes, out_axes)
         s1 = self.ex1.out_structure
         self.s_out = s_in
-        w1, o1 = _dense(w_left)   # (x, P, p, m)
-        w2, o2 = _dense(w_right)  # (m, Q, q, y)
+        w1, o1 = w_left[0], w_left[1]    # (x, P, p, m)
+        w2, o2 = w_right[0], w_right[1]  # (m, Q, q, y)
         w12 = np.einsum("xPpm,mQqy->xPQpqy", w1, w2)
         # t1 channels per (a, s) sector pair
         l1 = s1.layout
@@ -145,7 +187,7 @@ class _FusedChain:
                         dense = (o1[2][p] + ip, o2[2][q] + iq, o2[3][y] + iy)
                         by_as_in.setdefault((a, sg), []).append((off, dp * dq * my * ds, dense, da, ds))
         # t3 blocks: every (FL block (a,x,B)) x (output block (B,P,Q,s)) pair step 3 reads
-        lfl = fl.structure
+        lfl = fl_s
         out_by_b: dict = {}
         for j, key in enumerate(s_in.block_order):
             B, P, Q, sg = s_in.axis_sectors(key)
@@ -222,13 +264,13 @@ class _FusedChain:
             problems.append((db, dP * dQ * ds, int(ol.offsets[j]), dP * dQ * ds, segs_by_out.get(j, [])))
         self.gemm4 = GemmProgram(problems, lfl.size, self.t3_size, s_in.size, trans_a=True)
         self.flops = self.ex1.flops + self.gemm4.flops
-        self.fl, self.fr = fl, fr
 
-    def run(self, x: torch.Tensor, precision: str, events: dict | None = None) -> torch.Tensor:
+    def run(self, x: torch.Tensor, datas, precision: str, events: dict | None = None) -> torch.Tensor:
         from tenet import _native
 
+        fr_data, fl_data = datas
         ev1 = None if events is None else events.setdefault(0, [])
-        t1 = self.ex1.run(x, self.fr.data, precision, events=ev1)
+        t1 = self.ex1.run(x, fr_data, precision, events=ev1)
         t3 = torch.empty(self.t3_size, dtype=x.dtype, device=x.device)
         tb = self.mpo_tables.get(x.device)
         _native.check(_native.lib().tn_mpo_apply(
@@ -241,7 +283,7 @@ class _FusedChain:
         if events is not None:
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-        self.gemm4.run(self.fl.data, t3, out, precision)
+        self.gemm4.run(fl_data, t3, out, precision)
         if events is not None:
             e1.record()
             events.setdefault(3, []).append((e0, e1, self.gemm4.flops))
@@ -260,29 +302,34 @@ class Heff2:
         self.fl, self.fr = fl, fr
         dt = torch.float64 if precision == "fp64" else torch.float32
         self.dtype = dt
-        statics = []
-        for (eq, bend), t in zip(HEFF2_STEPS, (fr, w_right, w_left, fl)):
-            tb = eq.split("->")[0].split(",")[1]
-            if bend:
-                outs, ins, tb = _bent(t.structure, tb, bend)
-                t = ops.repartition(t, outs, ins)
-            statics.append((ops.astype(t, dt), tb))
-        self._statics = statics
         self._chains: dict = {}
-        self._graphs: dict = {}
         self.fused = bool(getattr(fl.provider, "unique_fusion", False)) if fused is None else fused
         if self.fused:
-            self._fl_w = ops.astype(fl, dt)
-            self._w = (w_left, w_right)
+            self._fr = ops.astype(fr, dt)
+            self._fl = ops.astype(fl, dt)
+            self._w = (_w_host(w_left), _w_host(w_right))
+            self._datas = (self._fr.data, self._fl.data)
+        else:
+            statics = []
+            for (eq, bend), t in zip(HEFF2_STEPS, (fr, w_right, w_left, fl)):
+                tb = eq.split("->")[0].split(",")[1]
+                if bend:
+                    outs, ins, tb = _bent(t.structure, tb, bend)
+                    t = ops.repartition(t, outs, ins)
+                statics.append((ops.astype(t, dt), tb))
+            self._statics = tuple((t.structure, tb) for t, tb in statics)
+            self._datas = tuple(t.data for t, _ in statics)
 
     def chain(self, s: TensorStructure):
         c = self._chains.get(s)
         if c is None:
             if self.fused:
-                c = _FusedChain(s, self._statics[0][0], self._fl_w, self._w[0], self._w[1],
-                                self._statics[0][1])
+                wl, wr = self._w
+                key = ("fused", s, self._fr.structure, self._fl.structure, wl[2], wr[2])
+                c = _cached_chain(key, lambda: _FusedChain(s, self._fr.structure, self._fl.structure,
+                                                           wl, wr, HEFF2_STEPS[0][0].split("->")[0].split(",")[1]))
             else:
-                c = _Chain(s, self._statics)
+                c = _cached_chain(("generic", s, self._statics), lambda: _Chain(s, self._statics))
             self._chains[s] = c
         return c
 
@@ -294,9 +341,10 @@ class Heff2:
         ``events`` (a dict) collects per-step GEMM events for kernel timing."""
         c = self.chain(structure)
         if x.dtype != self.dtype:
-            y = c.run(ops.astype(SymmetricTensor(structure, x), self.dtype).data, self.precision, events)
+            y = c.run(ops.astype(SymmetricTensor(structure, x), self.dtype).data, self._datas,
+                      self.precision, events)
             return ops.astype(SymmetricTensor(c.s_out, y), x.dtype).data
-        return c.run(x, self.precision, events)
+        return c.run(x, self._datas, self.precision, events)
 
     def __call__(self, aa: SymmetricTensor) -> SymmetricTensor:
         return SymmetricTensor(aa.structure, self.apply_data(aa.structure, aa.data))

@@ -27,7 +27,7 @@ from itertools import product
 
 import numpy as np
 
-from tenet.fusion_tree import FusionTree, coupled_sectors, fusion_trees
+from tenet.fusion_tree import FusionTree, _all_trees
 from tenet.space import OUT, Leg
 
 __all__ = ["FusionBlockKey", "TensorStructure"]
@@ -135,25 +135,31 @@ def _axes(s: TensorStructure):
     )
 
 
+def _side_trees(s: TensorStructure, axes) -> dict:
+    """Every fusion tree one side of ``s`` can carry, grouped by coupled sector."""
+    provider = s.provider
+    legs = [s.legs[i] for i in axes]
+    groups: dict = {}
+    for assignment in product(*(leg.sectors for leg in legs)):
+        fused = tuple(leg.fused_sector(a) for leg, a in zip(legs, assignment))
+        for t in _all_trees(provider, fused):
+            groups.setdefault(t.coupled, []).append(t)
+    return groups
+
+
 @cache
 def _block_order(s: TensorStructure) -> tuple:
+    """Sector matching: every (output tree, input tree) pair that meets in one coupled
+    sector, sorted.  Each side is enumerated once and the two are joined on the coupled
+    sector, so the cost is the two sides' sector products plus the number of blocks, not
+    the product over all legs."""
     p = _pattern(s)
     if p is not s:
         return _block_order(p)
-    provider = s.provider
     out_axes, in_axes = _axes(s)
-    keys = set()
-    for assignment in product(*(leg.sectors for leg in s.legs)):
-        fused = tuple(leg.fused_sector(a) for leg, a in zip(s.legs, assignment))
-        out_u = tuple(fused[i] for i in out_axes)
-        in_u = tuple(fused[i] for i in in_axes)
-        for c in coupled_sectors(provider, out_u):
-            ins = fusion_trees(provider, in_u, c)
-            if not ins:
-                continue
-            for ot in fusion_trees(provider, out_u, c):
-                for it in ins:
-                    keys.add(FusionBlockKey(ot, it))
+    outs = _side_trees(s, out_axes)
+    ins = _side_trees(s, in_axes)
+    keys = [FusionBlockKey(ot, it) for c, ots in outs.items() for ot in ots for it in ins.get(c, ())]
     return tuple(sorted(keys))
 
 
