@@ -25,7 +25,9 @@ from fractions import Fraction
 from functools import cache
 from math import factorial, sqrt
 
-__all__ = ["six_j", "f_symbol", "r_symbol", "b_symbol", "frobenius_schur"]
+import numpy as np
+
+__all__ = ["six_j", "f_symbol", "r_symbol", "b_symbol", "frobenius_schur", "cg", "z_matrix"]
 
 
 def _tri(a: int, b: int, c: int) -> bool:
@@ -108,3 +110,69 @@ def b_symbol(a: int, b: int, c: int) -> float:
 def frobenius_schur(a: int) -> float:
     """``chi_a = (-1)^{2ja}``: ``-1`` on half-integer spin."""
     return -1.0 if a % 2 else 1.0
+
+
+def _ladder(two_j: int, two_m: int, up: bool) -> float:
+    """``<j, m +- 1| J_+- |j, m>`` for doubled ``j`` and ``m`` (Condon-Shortley: positive)."""
+    j, m = two_j / 2.0, two_m / 2.0
+    return sqrt(max(j * (j + 1) - m * (m + 1 if up else m - 1), 0.0))
+
+
+@cache
+def _cg_cached(a: int, b: int, c: int) -> np.ndarray:
+    """Highest-weight construction of the ``(d_a, d_b, d_c)`` Clebsch-Gordan tensor.
+
+    The top state ``|c, c>`` is the normalised kernel of ``J_+ = J_+^a + J_+^b`` inside the
+    ``m = c`` subspace (a two-term recursion over ``m_a``), with the Condon-Shortley phase
+    ``<a, a; b, c-a | c, c> > 0``; every lower ``|c, m>`` follows by ``J_-``.  Magnetic
+    indices run descending (index ``k`` carries ``m = j - k``), the dense basis of
+    reference ``symmetry/su2.py:162-171``.
+    """
+    if not _tri(a, b, c):
+        raise ValueError(f"SU2: {c} is not in {a} x {b}")
+    da, db, dc = a + 1, b + 1, c + 1
+    out = np.zeros((da, db, dc))
+    # top state: amplitudes x[m_a] on |m_a, c - m_a> (doubled m), m_a descending from a
+    mas = [ma for ma in range(a, -a - 1, -2) if abs(c - ma) <= b]
+    amp = {mas[0]: 1.0}
+    for hi, lo in zip(mas, mas[1:]):
+        # J_+ kills the top state: coefficient of |hi, c - lo> must vanish
+        amp[lo] = -amp[hi] * _ladder(b, c - hi, True) / _ladder(a, lo, True)
+    nrm = sqrt(sum(v * v for v in amp.values()))
+    state = np.zeros((da, db))
+    for ma, v in amp.items():
+        state[(a - ma) // 2, (b - (c - ma)) // 2] = v / nrm
+    out[:, :, 0] = state
+    for k in range(1, dc):
+        m = c - 2 * (k - 1)  # lower |c, m> to |c, m - 1>
+        nxt = np.zeros((da, db))
+        for ia in range(da):
+            for ib in range(db):
+                v = state[ia, ib]
+                if v == 0.0:
+                    continue
+                ma, mb = a - 2 * ia, b - 2 * ib
+                if ia + 1 < da:
+                    nxt[ia + 1, ib] += v * _ladder(a, ma, False)
+                if ib + 1 < db:
+                    nxt[ia, ib + 1] += v * _ladder(b, mb, False)
+        state = nxt / _ladder(c, m, False)
+        out[:, :, k] = state
+    out.flags.writeable = False
+    return out
+
+
+def cg(a: int, b: int, c: int) -> np.ndarray:
+    """``<a m_a; b m_b | c m_c>`` as a read-only ``(d_a, d_b, d_c)`` array (doubled spins)."""
+    return _cg_cached(a, b, c)
+
+
+@cache
+def z_matrix(a: int) -> np.ndarray:
+    """``Z_a: V_a -> V_a^*`` in the descending basis: ``Z[i, d-1-i] = (-1)^i``."""
+    d = a + 1
+    z = np.zeros((d, d))
+    for i in range(d):
+        z[i, d - 1 - i] = -1.0 if i % 2 else 1.0
+    z.flags.writeable = False
+    return z

@@ -83,7 +83,13 @@ class _Chain:
         for k, ((rep, s, ex), data) in enumerate(zip(self.steps, datas)):
             if rep is not None:
                 nxt = torch.empty(s.size, dtype=cur.dtype, device=cur.device)
+                if events is not None:
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record()
                 rep.run(cur, nxt)
+                if events is not None:
+                    e1.record()
+                    events.setdefault(f"recouple{k}", []).append((e0, e1, rep.bytes_moved(cur.element_size())))
                 cur = nxt
             ev = None if events is None else events.setdefault(k, [])
             cur = ex.run(cur, data, precision, events=ev)
@@ -303,7 +309,7 @@ class Heff2:
         dt = torch.float64 if precision == "fp64" else torch.float32
         self.dtype = dt
         self._chains: dict = {}
-        self.fused = bool(getattr(fl.provider, "unique_fusion", False)) if fused is None else fused
+        self.fused = bool(getattr(fl.provider, "bosonic", False)) if fused is None else fused
         if self.fused:
             self._fr = ops.astype(fr, dt)
             self._fl = ops.astype(fl, dt)

@@ -77,37 +77,13 @@ class MPS:
         return self
 
 
-def from_dense(array, legs: Sequence[Leg], *, atol: float = 1e-12, device=None) -> SymmetricTensor:
+def from_dense(array, legs: Sequence[Leg], *, atol: float = 1e-10, device=None) -> SymmetricTensor:
     """Blocks of a dense array whose every axis runs over its leg's sectors in canonical
-    order (abelian providers: no Clebsch-Gordan factor).  Raises when the array has weight
-    outside the symmetric block set."""
-    s = TensorStructure(tuple(legs))
-    a = np.asarray(array, dtype=np.float64)
-    offs = []
-    for leg in s.legs:
-        if not getattr(leg.provider, "unique_fusion", False):
-            raise NotImplementedError("from_dense is implemented for abelian providers")
-        o, acc = {}, 0
-        for sec, m in leg.space.sectors:
-            o[sec] = acc
-            acc += m
-        offs.append(o)
-        if acc != a.shape[len(offs) - 1]:
-            raise ValueError(f"axis {len(offs) - 1} has dimension {a.shape[len(offs) - 1]}, leg needs {acc}")
-    flat = np.empty(s.size)
-    check = np.zeros_like(a)
-    lay = s.layout
-    for i, key in enumerate(s.block_order):
-        secs = s.axis_sectors(key)
-        sl = tuple(slice(o[x], o[x] + leg.degeneracy(x)) for o, leg, x in zip(offs, s.legs, secs))
-        blk = a[sl]
-        flat[int(lay.offsets[i]) : int(lay.offsets[i] + lay.sizes[i])] = blk.reshape(-1)
-        check[sl] = blk
-    resid = float(np.abs(a - check).max()) if a.size else 0.0
-    if resid > atol * max(1.0, float(np.abs(a).max())):
-        raise ValueError(f"from_dense: the array is not symmetric on these legs (residual {resid:.3g})")
-    dev = torch.device(device) if device is not None else default_device()
-    return SymmetricTensor(s, torch.from_numpy(flat).to(dev))
+    order (``tenet.dense.from_dense``: Clebsch-Gordan projection for SU(2), block
+    placement for abelian providers).  Raises when the array is not symmetric."""
+    from tenet.dense import from_dense as _from_dense
+
+    return _from_dense(array, legs, atol=atol, device=device)
 
 
 class MPO:

@@ -15,7 +15,7 @@ from tenet.network.env import Env
 from tenet.network.models import chain_bonds, heisenberg_mpo, PHYS
 from tenet.network.mps import MPS
 
-__all__ = ["heff2_problem"]
+__all__ = ["heff2_problem", "su2_heff2_problem"]
 
 
 def heff2_problem(n_sites: int = 64, max_dim: int = 2048, sigma: float = 6.0, seed: int = 0,
@@ -37,3 +37,32 @@ def heff2_problem(n_sites: int = 64, max_dim: int = 2048, sigma: float = 6.0, se
     aa = ops.einsum("apx,xqr->apqr", psi[c], psi[c + 1])
     aa = ops.divide(aa, ops.norm(aa))
     return env.F[c - 1, c], h[c], h[c + 1], env.F[c + 2, c + 1], aa
+
+
+def su2_heff2_problem(n_sites: int = 64, max_mult: int = 512, sigma: float = 6.0, seed: int = 0,
+                      device="cuda"):
+    """The two-site H_eff of the SU(2) Heisenberg chain at its centre bond: bonds of
+    ``max_mult`` multiplets with a Gaussian profile in ``2j`` (at most ``2j = 12`` on even
+    bonds, ``13`` on odd ones), random N(0,1) sites each normalised, environments folded
+    from them and normalised.  Returns ``(fl, w_left, w_right, fr, aa)`` in float64."""
+    from tenet.network.models import SU2_PHYS, su2_chain_bonds, su2_heisenberg_mpo
+
+    bonds = su2_chain_bonds(n_sites, max_mult, sigma, max_two_j=12)
+    psi = MPS.random(SU2_PHYS, bonds, seed=seed, device=device)
+    for i in range(n_sites):
+        psi.sites[i] = ops.divide(psi[i], ops.norm(psi[i]))
+    h = su2_heisenberg_mpo(n_sites, device=device)
+    env = Env(psi, h)
+    c = n_sites // 2 - 1
+    for n in range(0, c):
+        env.update_(n, to="last")
+        f = env.F[n, n + 1]
+        env.F[n, n + 1] = ops.divide(f, ops.norm(f))
+    for n in range(n_sites - 1, c + 1, -1):
+        env.update_(n, to="first")
+        f = env.F[n, n - 1]
+        env.F[n, n - 1] = ops.divide(f, ops.norm(f))
+    aa = ops.einsum("apx,xqr->apqr", psi[c], psi[c + 1])
+    aa = ops.divide(aa, ops.norm(aa))
+    return env.F[c - 1, c], h[c], h[c + 1], env.F[c + 2, c + 1], aa
+

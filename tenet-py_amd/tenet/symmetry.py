@@ -9,6 +9,13 @@ Three providers cover the contraction engine's workloads:
 * ``SU2``      -- labels are ``2j``; triangle fusion, ``qdim = 2j+1``, F from the Racah
   6j formula in the TensorKit gauge, ``R = (-1)^(ja+jb-jc)``, ``B`` derived from F and
   ``chi = (-1)^(2j)`` (reference ``symmetry/su2.py:150-252``, ``_sun_coeff.py:75-117``).
+* ``fZ2``      -- fermion parity: XOR fusion, F = B = FS = 1, and the whole fermionic
+  content in the braid: a permutation of a tree's lines costs the Koszul sign
+  ``(-1)^(number of odd-odd inversions)`` (reference ``symmetry/fz2.py:85-130,185-200``).
+
+Every provider also carries its dense-basis data (``irrep_dim``, ``cgc``, ``z_matrix``)
+for ``to_dense``/``from_dense``.  ``bosonic`` says that every braid and bend coefficient
+is exactly 1, which is what lets the two-site H_eff run as a pure index computation.
 
 Coefficients are host-side plan data: they enter only when a plan is compiled, never per
 block element on the device.
@@ -19,6 +26,8 @@ from __future__ import annotations
 from dataclasses import dataclass
 from functools import cache
 from math import sqrt
+
+import numpy as np
 from typing import TYPE_CHECKING
 
 from tenet import _su2coeff
@@ -34,9 +43,12 @@ __all__ = [
     "TrivialSector",
     "U1Sector",
     "SU2Sector",
+    "FZ2Sector",
     "Trivial",
     "U1",
     "SU2",
+    "fZ2",
+    "koszul_sign",
 ]
 
 
@@ -78,6 +90,19 @@ class SU2Sector(Sector):
             raise ValueError(f"two_j must be non-negative, got {self.two_j}")
 
 
+@dataclass(frozen=True, slots=True, order=True)
+class FZ2Sector(Sector):
+    """A fermion-parity sector: ``parity`` 0 (even, the unit) or 1 (odd)."""
+
+    parity: int
+
+    def __post_init__(self) -> None:
+        if not isinstance(self.parity, int) or isinstance(self.parity, bool):
+            raise TypeError(f"parity must be an int, got {self.parity!r}")
+        if self.parity not in (0, 1):
+            raise ValueError(f"parity must be 0 or 1, got {self.parity}")
+
+
 # --------------------------------------------------------------------------------------
 # Unique-fusion helpers (Trivial, U1): one term, coefficient exactly 1.
 # Reference: ``permute_unique_tree`` and ``bend_unique`` in ``symmetry/base.py:778-1074``.
@@ -97,9 +122,23 @@ def _only_tree(provider, uncoupled, coupled) -> "FusionTree":
     return trees[0]
 
 
-def _permute_unique(provider, tree, perm):
+def _permute_unique(provider, tree, perm, coeff: float = 1.0):
     uncoupled = tuple(tree.uncoupled[i] for i in perm)
-    return ((_only_tree(provider, uncoupled, tree.coupled), 1.0),)
+    return ((_only_tree(provider, uncoupled, tree.coupled), coeff),)
+
+
+def koszul_sign(uncoupled, perm) -> float:
+    """``(-1)^(number of inversions of perm among odd lines)``; ``perm[j]`` is the old
+    position that becomes position ``j`` (reference ``symmetry/fz2.py:85-130``)."""
+    odd = [i for i in perm if uncoupled[i].parity]
+    inv = sum(1 for j in range(len(odd)) for k in range(j + 1, len(odd)) if odd[j] > odd[k])
+    return -1.0 if inv % 2 else 1.0
+
+
+_ONE_CGC = np.ones((1, 1, 1))
+_ONE_CGC.flags.writeable = False
+_ONE_Z = np.ones((1, 1))
+_ONE_Z.flags.writeable = False
 
 
 def _bend_unique(provider, key, right: bool):
@@ -145,6 +184,19 @@ class TrivialProvider:
     def frobenius_schur(self, a) -> float:
         return 1.0
 
+    def irrep_dim(self, a) -> int:
+        return 1
+
+    def cgc(self, a, b, c) -> np.ndarray:
+        return _ONE_CGC
+
+    def z_matrix(self, a) -> np.ndarray:
+        return _ONE_Z
+
+    @property
+    def bosonic(self) -> bool:
+        return True
+
     def permute_tree(self, tree, perm):
         return _permute_unique(self, tree, perm)
 
@@ -186,6 +238,19 @@ class U1Provider:
 
     def frobenius_schur(self, a) -> float:
         return 1.0
+
+    def irrep_dim(self, a) -> int:
+        return 1
+
+    def cgc(self, a, b, c) -> np.ndarray:
+        return _ONE_CGC
+
+    def z_matrix(self, a) -> np.ndarray:
+        return _ONE_Z
+
+    @property
+    def bosonic(self) -> bool:
+        return True
 
     def permute_tree(self, tree, perm):
         return _permute_unique(self, tree, perm)
@@ -247,6 +312,20 @@ class SU2Provider:
     def frobenius_schur(self, a: SU2Sector) -> float:
         return _su2coeff.frobenius_schur(a.two_j)
 
+    def irrep_dim(self, a: SU2Sector) -> int:
+        return a.two_j + 1
+
+    def cgc(self, a: SU2Sector, b: SU2Sector, c: SU2Sector) -> np.ndarray:
+        """``(d_a, d_b, d_c)`` Clebsch-Gordan tensor, magnetic indices descending."""
+        return _su2coeff.cg(a.two_j, b.two_j, c.two_j)
+
+    def z_matrix(self, a: SU2Sector) -> np.ndarray:
+        return _su2coeff.z_matrix(a.two_j)
+
+    @property
+    def bosonic(self) -> bool:
+        return False
+
     def permute_tree(self, tree, perm):
         return _permute_braided(self, tree, tuple(perm))
 
@@ -258,6 +337,63 @@ class SU2Provider:
 
     @property
     def unique_fusion(self) -> bool:
+        return False
+
+
+@dataclass(frozen=True, slots=True)
+class FZ2Provider:
+    """Fermion parity: Z2 fusion with fermionic braiding.  Abelian, ``d = qdim = 1``;
+    ``F = B = FS = 1``, so a bend costs nothing and a permutation costs the Koszul sign
+    of the odd lines it moves past each other (reference ``symmetry/fz2.py:133-240``)."""
+
+    name: str = "fZ2"
+
+    @property
+    def unit(self) -> FZ2Sector:
+        return FZ2Sector(0)
+
+    def dual(self, a: FZ2Sector) -> FZ2Sector:
+        return a
+
+    def fusion(self, a: FZ2Sector, b: FZ2Sector) -> tuple[FZ2Sector, ...]:
+        return (FZ2Sector(a.parity ^ b.parity),)
+
+    def n_symbol(self, a, b, c) -> int:
+        return int(c.parity == a.parity ^ b.parity)
+
+    def qdim(self, a) -> float:
+        return 1.0
+
+    def frobenius_schur(self, a) -> float:
+        return 1.0
+
+    def twist(self, a: FZ2Sector) -> float:
+        return -1.0 if a.parity else 1.0
+
+    def irrep_dim(self, a) -> int:
+        return 1
+
+    def cgc(self, a, b, c) -> np.ndarray:
+        return _ONE_CGC
+
+    def z_matrix(self, a) -> np.ndarray:
+        return _ONE_Z
+
+    def permute_tree(self, tree, perm):
+        return _permute_unique(self, tree, perm, koszul_sign(tree.uncoupled, perm))
+
+    def bend_right(self, key, *, dual: bool):
+        return _bend_unique(self, key, True)
+
+    def bend_left(self, key, *, dual: bool):
+        return _bend_unique(self, key, False)
+
+    @property
+    def unique_fusion(self) -> bool:
+        return True
+
+    @property
+    def bosonic(self) -> bool:
         return False
 
 
@@ -356,3 +492,4 @@ def _bend_braided(provider, key, right: bool, dual: bool):
 Trivial = TrivialProvider()
 U1 = U1Provider()
 SU2 = SU2Provider()
+fZ2 = FZ2Provider()
