@@ -34,6 +34,17 @@ sys.path.insert(0, str(ROOT / "tenet-py_amd"))
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16x3": 2500.0}
 
+# HBM traffic per launch of the H_eff GEMM kernels (keys: the events' step numbers), read
+# from `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` (KiB per dispatch) of
+# `bench.py --steps 20 --warmup 3 --no-cpu --no-blocks --no-energy --arms fp32`, and
+# scaled by what the same counters read for `bench.py --workload copy` (a 1 GiB device
+# copy: FETCH_SIZE 524302 KiB for 1048576 KiB read, WRITE_SIZE 1048576 KiB written).
+PMC_TRAFFIC = {
+    (64, 2048, "fp32"): {"fetch_kib": {0: 56556.6, 3: 100152.4}, "write_kib": {0: 33081.0, 3: 19803.7},
+                         "fetch_scale": 0.5, "write_scale": 1.0,
+                         "source": "profiles/heff2_L64_D2048_fp32_traffic.md"},
+}
+
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -53,12 +64,14 @@ def time_arm(fn, steps: int, warmup: int):
 
 
 def restarted_lanczos(op, aa, refine=None, max_restarts: int = 40, ncv: int = 8, tol: float = 1e-14):
-    """Thick-free restarted Lanczos to convergence of the Ritz value; returns (E, matvecs)."""
-    from tenet.network import lanczos
+    """Thick-free restarted Lanczos to convergence of the Ritz value, each restart replaying
+    the captured hipGraphs of its iterations; returns (E, matvecs)."""
+    from tenet.network.graph import KrylovGraph
 
+    kg = KrylovGraph(op, aa.structure, ncv=ncv)
     v, e, used = aa, None, 0
     for _ in range(max_restarts):
-        e_new, v = lanczos(op, v, ncv=ncv, refine=refine)
+        e_new, v = kg.solve(v, refine=refine)
         used += ncv + (1 if refine else 0)
         if e is not None and abs(e_new - e) <= tol * max(1.0, abs(e_new)):
             e = e_new
@@ -98,6 +111,7 @@ def main() -> int:
     ap.add_argument("--bond", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-blocks", action="store_true")
+    ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--arms", default="fp32,bf16x3,fp64")
     ap.add_argument("--workload", default="heff2",
                     choices=["heff2", "einsum", "batched", "dmrg", "serve", "su2", "copy"])
@@ -136,6 +150,7 @@ def bench_heff2(args) -> int:
     import tenet
     from tenet.baseline_blocks import BlockTensor, heff2 as blocks_heff2
     from tenet.network import Heff2
+    from tenet.network.graph import GraphedMatvec
     from tenet.network.workloads import heff2_problem
 
     dev = torch.device("cuda")
@@ -184,31 +199,53 @@ def bench_heff2(args) -> int:
         ops_[head].apply_data(aa.structure, aa.data, events=ev)
     torch.cuda.synchronize()
     per_step = {k: (sum(e0.elapsed_time(e1) for e0, e1, _ in v) / len(v), v[0][2]) for k, v in ev.items()
-                if not str(k).startswith("recouple")}
-    dom = max(per_step, key=lambda k: per_step[k][1])
+                if v and not str(k).startswith(("recouple", "reduce"))}
+    reduce_ms = {str(k): sum(e0.elapsed_time(e1) for e0, e1, _ in v) / len(v) for k, v in ev.items()
+                 if v and str(k).startswith("reduce")}
     big = [k for k in per_step if per_step[k][1] >= 0.05 * flops]
     gemm_ms = sum(per_step[k][0] for k in big)
     gemm_flops = sum(per_step[k][1] for k in big)
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
+    pmc = PMC_TRAFFIC.get((args.sites, args.bond, head))
+    traffic = None
+    if pmc is not None:
+        traffic = sum(pmc["fetch_kib"][k] / pmc["fetch_scale"] + pmc["write_kib"][k] / pmc["write_scale"]
+                      for k in big) * 1024.0
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS[head], "traffic": None,
-                "kernel": f"tn_gemm_grouped ({head}) H_eff steps {sorted(big)}",
+                "frac": achieved / PEAK_TFLOPS[head], "traffic": traffic,
+                "kernel": f"gemm_kernel (tn_gemm_grouped, {head}): the two H_eff GEMM launches of one "
+                          f"matvec, steps {sorted(big)}; achieved = their flops / their summed average "
+                          "kernel durations, traffic = their summed HBM bytes",
                 "avg_launch_ms": {str(k): per_step[k][0] for k in per_step},
-                "flops_per_launch": {str(k): per_step[k][1] for k in per_step}}
+                "flops_per_launch": {str(k): per_step[k][1] for k in per_step},
+                "split_k_reduce_avg_ms": reduce_ms,
+                "traffic_source": pmc["source"] if pmc else None}
 
-    # energy agreement: FP32/BF16x3 Krylov + FP64 Rayleigh quotient vs all-FP64
-    e64, n64 = restarted_lanczos(ops_["fp64"], aa)
-    hv = ops_["fp64"](aa)
-    scale = float(tenet.norm(hv)) / float(tenet.norm(aa))  # a lower bound of |H|
-    energy = {"E_fp64": e64, "matvecs_fp64": n64, "operator_scale": scale}
-    for p in arms:
-        if p == "fp64":
-            continue
-        ep, npv = restarted_lanczos(ops_[p], aa, refine=ops_["fp64"])
-        energy[f"E_{p}"] = ep
-        energy[f"matvecs_{p}"] = npv
-        energy[f"dE_rel_{p}"] = abs(ep - e64) / abs(e64)
-        energy[f"dE_rel_scale_{p}"] = abs(ep - e64) / scale
+    # the headline arm's matvec captured as a hipGraph and replayed on resident buffers
+    gm = GraphedMatvec(ops_[head], aa.structure)
+    gm.x.copy_(aa.data)
+    dt = time_arm(gm.graph.replay, args.steps, args.warmup)
+    gerr = float((gm.y - ref).norm() / ref.norm())
+    results[f"{head}_graph"] = {"matvecs_per_s": 1.0 / dt, "ms": dt * 1e3,
+                                "effective_tflops": flops / dt / 1e12, "matvec_rel_err_vs_fp64": gerr}
+    log(f"arm {head}_graph: {dt * 1e3:.3f} ms/matvec, {1 / dt:.1f} matvecs/s")
+
+    # energy agreement: FP32/BF16x3 Krylov + FP64 Rayleigh quotient vs all-FP64, each a
+    # restarted Lanczos whose iterations replay captured hipGraphs
+    energy = {}
+    if not args.no_energy:
+        e64, n64 = restarted_lanczos(ops_["fp64"], aa)
+        hv = ops_["fp64"](aa)
+        scale = float(tenet.norm(hv)) / float(tenet.norm(aa))  # a lower bound of |H|
+        energy = {"E_fp64": e64, "matvecs_fp64": n64, "operator_scale": scale}
+        for p in arms:
+            if p == "fp64":
+                continue
+            ep, npv = restarted_lanczos(ops_[p], aa, refine=ops_["fp64"])
+            energy[f"E_{p}"] = ep
+            energy[f"matvecs_{p}"] = npv
+            energy[f"dE_rel_{p}"] = abs(ep - e64) / abs(e64)
+            energy[f"dE_rel_scale_{p}"] = abs(ep - e64) / scale
     log(f"energies {energy}")
 
     cpu = None
@@ -219,7 +256,9 @@ def bench_heff2(args) -> int:
         results["fp64"]["rel_err_vs_oracle"] = float(np.linalg.norm(got64 - ref_cpu) / np.linalg.norm(ref_cpu))
         log(f"cpu baseline {cpu['value']:.3f} matvecs/s")
 
-    v = results[head]["matvecs_per_s"]
+    # the headline is the faster of the two launch modes of the same kernels
+    best = f"{head}_graph" if results[f"{head}_graph"]["matvecs_per_s"] > results[head]["matvecs_per_s"] else head
+    v = results[best]["matvecs_per_s"]
     line = {
         "metric": "H_eff matvecs/s FP32-MFMA vs FP64 at |dE|<1e-8",
         "value": v,
@@ -227,7 +266,7 @@ def bench_heff2(args) -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": results[head]["ms"],
+        "ms_per_step": results[best]["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -238,6 +277,7 @@ def bench_heff2(args) -> int:
                         f"(22 odd-charge sectors, Gaussian sigma=6), MPO bond 5, phys dim 2",
             "aa_elements": aa.structure.size,
             "flops_per_matvec": flops,
+            "launch": "hipGraph replay of the captured matvec" if best != head else "eager launches",
         },
         "arms": results,
         "speedup_fp32_vs_fp64_blocks": (v / results["fp64_blocks"]["matvecs_per_s"]) if "fp64_blocks" in results else None,

@@ -426,7 +426,11 @@ class GemmProgram:
         return d.get(device), d.ntiles
 
     def run(self, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, precision: str | None = None,
-            tile=None, grid: int = 0) -> None:
+            tile=None, grid: int = 0, events: list | None = None, reduce_events: list | None = None) -> None:
+        """One grouped launch (plus the split-K reduction when the program has one).  With
+        ``events`` a list, the GEMM kernel alone is bracketed by two recorded events,
+        appended as ``(start, end, flops)``; ``reduce_events`` does the same for the
+        reduction with its algorithmic bytes."""
         if self.nproblems == 0:
             return
         if not (A.dtype == B.dtype == C.dtype):
@@ -443,6 +447,9 @@ class GemmProgram:
             # the dispatcher's dynamic balancing), so it is opt-in.
             grid = min(ntiles, resident_workgroups(prec, tile))
         W = torch.empty(self.ws_size, dtype=C.dtype, device=C.device) if self.ws_size else C
+        if events is not None:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
         _native.check(
             _native.lib().tn_gemm_grouped(
                 _native.PRECISIONS[prec],
@@ -463,8 +470,17 @@ class GemmProgram:
                 _native.stream_ptr(),
             )
         )
+        if events is not None:
+            e1.record()
+            events.append((e0, e1, self.flops))
         if self.reduce is not None:
+            if reduce_events is not None:
+                r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                r0.record()
             self.reduce.run(W, C)
+            if reduce_events is not None:
+                r1.record()
+                reduce_events.append((r0, r1, self.reduce.bytes_moved(C.element_size())))
 
 
 # ----------------------------------------------------------------------------- contraction
@@ -562,9 +578,11 @@ class ContractionExec:
         _ = n_out_b
 
     def run(self, a_data: torch.Tensor, b_data: torch.Tensor, precision=None, tile=None,
-            out: torch.Tensor | None = None, events: list | None = None) -> torch.Tensor:
-        """Run the contraction; when ``events`` is a list, the GEMM launch is bracketed by
-        two recorded events appended as ``(start, end, flops)``."""
+            out: torch.Tensor | None = None, events: list | None = None,
+            reduce_events: list | None = None) -> torch.Tensor:
+        """Run the contraction; when ``events`` is a list, the GEMM kernel launch is
+        bracketed by two recorded events appended as ``(start, end, flops)`` (and a split-K
+        reduction, if any, into ``reduce_events``)."""
         dtype, dev = a_data.dtype, a_data.device
         if b_data.dtype != dtype:
             raise TypeError("tensordot: operands have different dtypes")
@@ -582,13 +600,7 @@ class ContractionExec:
             out = torch.empty(self.out_structure.size, dtype=dtype, device=dev)
         target = out if self.c_program is None else torch.empty(
             self.c_structure.size, dtype=dtype, device=dev)
-        if events is not None:
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-        self.gemm.run(ap, bp, target, precision, tile)
-        if events is not None:
-            e1.record()
-            events.append((e0, e1, self.flops))
+        self.gemm.run(ap, bp, target, precision, tile, events=events, reduce_events=reduce_events)
         if self.c_program is not None:
             self.c_program.run(target, out)
         return out

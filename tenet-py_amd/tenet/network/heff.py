@@ -92,7 +92,8 @@ class _Chain:
                     events.setdefault(f"recouple{k}", []).append((e0, e1, rep.bytes_moved(cur.element_size())))
                 cur = nxt
             ev = None if events is None else events.setdefault(k, [])
-            cur = ex.run(cur, data, precision, events=ev)
+            rev = None if events is None else events.setdefault(f"reduce{k}", [])
+            cur = ex.run(cur, data, precision, events=ev, reduce_events=rev)
         return cur
 
 
@@ -276,7 +277,8 @@ class _FusedChain:
 
         fr_data, fl_data = datas
         ev1 = None if events is None else events.setdefault(0, [])
-        t1 = self.ex1.run(x, fr_data, precision, events=ev1)
+        rev1 = None if events is None else events.setdefault("reduce0", [])
+        t1 = self.ex1.run(x, fr_data, precision, events=ev1, reduce_events=rev1)
         t3 = torch.empty(self.t3_size, dtype=x.dtype, device=x.device)
         tb = self.mpo_tables.get(x.device)
         _native.check(_native.lib().tn_mpo_apply(
@@ -286,13 +288,9 @@ class _FusedChain:
             (tb["coef64"] if x.dtype == torch.float64 else tb["coef32"]).data_ptr(),
             tb["tiles"].data_ptr(), self.mpo_ntiles, _native.stream_ptr()))
         out = torch.empty(self.s_out.size, dtype=x.dtype, device=x.device)
-        if events is not None:
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-        self.gemm4.run(fl_data, t3, out, precision)
-        if events is not None:
-            e1.record()
-            events.setdefault(3, []).append((e0, e1, self.gemm4.flops))
+        self.gemm4.run(fl_data, t3, out, precision,
+                       events=None if events is None else events.setdefault(3, []),
+                       reduce_events=None if events is None else events.setdefault("reduce3", []))
         return out
 
 
