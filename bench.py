@@ -72,7 +72,7 @@ def restarted_lanczos(op, aa, refine=None, max_restarts: int = 40, ncv: int = 8,
     v, e, used = aa, None, 0
     for _ in range(max_restarts):
         e_new, v = kg.solve(v, refine=refine)
-        used += ncv + (1 if refine else 0)
+        used += ncv + (2 if refine else 0)
         if e is not None and abs(e_new - e) <= tol * max(1.0, abs(e_new)):
             e = e_new
             break

@@ -145,6 +145,19 @@ int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int64_t nwgs, 
 int tn_svd_jacobi_finish(const int64_t* probs, int64_t nprobs, const double* x, const double* q,
                          double* U, double* S, double* Vh, void* stream);
 
+/* Ground eigenpair of the Lanczos tridiagonal matrix, on the device: replaces the
+ * np.linalg.eigh(tri) of network/dmrg.py:216-222 (lanczos) and the host read of the Krylov
+ * scalars before it (:212).  alpha[0..n) is the diagonal, beta[0..n-1) the off-diagonal;
+ * the matrix is truncated at the first |beta[j]| < tol (happy breakdown, :187-190).
+ * Writes value[0] (lowest eigenvalue) and coef[0..n) (its unit eigenvector, zero past the
+ * truncation, first nonzero entry positive).  One workgroup; n <= TN_TRIDIAG_MAX.
+ * tn_tridiag_ground_host runs the same routine on the host (host pointers, no stream). */
+#define TN_TRIDIAG_MAX 64
+int tn_tridiag_ground(const double* alpha, const double* beta, int n, double tol, double* value,
+                      double* coef, void* stream);
+int tn_tridiag_ground_host(const double* alpha, const double* beta, int n, double tol,
+                           double* value, double* coef);
+
 #ifdef __cplusplus
 }
 #endif

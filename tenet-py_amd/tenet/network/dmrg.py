@@ -2,14 +2,16 @@
 device.
 
 ``lanczos`` follows reference ``network/dmrg.py:106-227`` (three-term recurrence, happy
-breakdown below ``tol``, ``eigh`` of the small tridiagonal, one recombination) with two
-MI355X-side changes that do not change the mathematics: every Krylov scalar
-(``alpha_j = <v_j|w>``, ``beta_j = |w|``) stays in device memory and the host reads them
-once, after the recurrence; and each new vector is fully re-orthogonalized in FP64
-against the previous ones (the reference's ncv=3 recurrence is orthogonal in exact
-arithmetic; this keeps it so when the matvec is FP32).  ``refine`` takes an FP64 matvec:
-the returned value is then the FP64 Rayleigh quotient of the Ritz vector, which removes
-the first-order error of an FP32 operator from the energy.
+breakdown below ``tol``, the ground pair of the small tridiagonal, one recombination)
+with MI355X-side changes that do not change the mathematics: every Krylov scalar
+(``alpha_j = <v_j|w>``, ``beta_j = |w|``) stays in device memory, the tridiagonal
+eigenproblem and the recombination run on the device too (``ritz``), so the host reads
+one number -- the energy -- per Lanczos step; and each new vector is fully
+re-orthogonalized in FP64 against the previous ones (the reference's ncv=3 recurrence is
+orthogonal in exact arithmetic; this keeps it so when the matvec is FP32).  ``refine``
+takes an FP64 matvec and closes the step with ``residual_refine``: a Rayleigh-Ritz
+correction in the span of the Ritz vector and its FP64 residual, which removes the FP32
+operator's error from the energy and the vector.
 
 ``sweep_``/``dmrg_`` follow reference ``dmrg.py:321-843`` (noiseless and wavefunction-noise
 splits; perturbative noise is not part of this build).
@@ -91,24 +93,60 @@ def lanczos(matvec: Callable[[SymmetricTensor], SymmetricTensor], v: SymmetricTe
         if j + 1 == ncv:
             break
         vecs.append(ops.divide(w, betas[j]))
-    a_h = alphas.cpu().numpy()
-    b_h = betas.cpu().numpy()
-    m = ncv
-    for j in range(ncv - 1):
-        if b_h[j] < tol:  # happy breakdown: drop the rest, keep the space
-            m = j + 1
-            break
-    tri = np.diag(a_h[:m]) + np.diag(b_h[: m - 1], 1) + np.diag(b_h[: m - 1], -1)
-    values, states = np.linalg.eigh(tri)
-    g = states[:, 0]
-    out = ops.multiply(vecs[0], float(g[0]))
-    for k in range(1, m):
-        out = ops.add(out, ops.multiply(vecs[k], float(g[k])))
+    value, out = ritz(alphas, betas, [x.data for x in vecs], tol)
+    out = SymmetricTensor(v.structure, out)
     out = ops.divide(out, ops.norm(out))
-    value = float(values[0])
     if refine is not None:
-        value = float(ops.inner(out, refine(out)))
+        value, out, _ = residual_refine(refine, out)
+    return float(value), out
+
+
+def ritz(alphas: torch.Tensor, betas: torch.Tensor, basis, tol: float, coef=None, value=None, out=None):
+    """Ground Ritz pair of a Lanczos recurrence, on the device: the tridiagonal eigenproblem
+    (``tn_tridiag_ground``, happy breakdown at the first ``beta < tol``) and the
+    recombination ``sum_k g_k v_k``, normalised.  Returns ``(value, vector)`` as device
+    tensors; nothing is read back to the host.  ``coef``/``value``/``out`` are optional
+    preallocated buffers (the captured Krylov graphs pass theirs)."""
+    from tenet import _native
+
+    n = len(basis)
+    dev = alphas.device
+    coef = torch.empty(n, dtype=torch.float64, device=dev) if coef is None else coef
+    value = torch.empty((), dtype=torch.float64, device=dev) if value is None else value
+    _native.check(_native.lib().tn_tridiag_ground(alphas.data_ptr(), betas.data_ptr(), n, tol,
+                                                  value.data_ptr(), coef.data_ptr(), _native.stream_ptr()))
+    out = torch.empty_like(basis[0]) if out is None else out
+    lib = _native.lib()
+    for k, x in enumerate(basis):
+        # out = g_k x + (k ? out : 0); entries of g past the breakdown are exactly zero
+        _native.check(lib.tn_axpby(_native.TN_F64, x.numel(), 1.0, coef[k:].data_ptr(), 0, x.data_ptr(),
+                                   1.0 if k else 0.0, None, 0, out.data_ptr(), _native.stream_ptr()))
     return value, out
+
+
+def residual_refine(op64: Callable[[SymmetricTensor], SymmetricTensor], psi: SymmetricTensor):
+    """The closing FP64 residual-refinement step of a reduced-precision Krylov solve.
+
+    With ``w = H psi`` (FP64 operator), ``E = <psi|H|psi>``, the residual ``r = w - E psi``
+    is orthogonal to ``psi``; the Rayleigh-Ritz problem of ``H`` on ``span{psi, r/|r|}`` is
+    the 2x2 tridiagonal ``[[E, |r|], [|r|, <u|H|u>]]`` whose ground pair (solved on the
+    device like the Lanczos one) gives the refined energy and vector.  The energy is never
+    above the Rayleigh quotient of ``psi``, and an FP32 operator's first-order error is
+    gone from it.  Returns ``(energy, vector, |r|)`` with device scalars.  Costs two FP64
+    matvecs."""
+    psi = ops.divide(psi, ops.norm(psi))
+    w = op64(psi)
+    e = ops.inner_device(psi, w)
+    r = ops.subtract(w, ops.multiply(psi, e))
+    rho = ops.norm(r)
+    u = ops.divide(r, torch.clamp(rho, min=1e-300))  # r == 0 leaves u == 0 and g = (1, 0)
+    hu = op64(u)
+    euu = ops.inner_device(u, hu)
+    alphas = torch.stack((e, euu))
+    betas = torch.stack((rho, torch.zeros_like(rho)))
+    value, out = ritz(alphas, betas, [psi.data, u.data], 1e-300)
+    out = SymmetricTensor(psi.structure, out)
+    return value, ops.divide(out, ops.norm(out)), rho
 
 
 def spectrum(s: SymmetricTensor) -> list:

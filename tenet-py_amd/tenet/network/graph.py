@@ -23,7 +23,6 @@ their buffers: it belongs to one compiled ``Heff2`` and one vector structure.
 
 from __future__ import annotations
 
-import numpy as np
 import torch
 
 from tenet import _native, ops
@@ -67,13 +66,17 @@ class GraphedMatvec:
 
 
 class KrylovGraph:
-    """``ncv`` Lanczos iterations of ``op`` on vectors of ``structure``, one graph each."""
+    """One Lanczos step of ``op`` on vectors of ``structure``: ``ncv`` iteration graphs and
+    one finishing graph (device tridiagonal solve, recombination, normalisation)."""
 
-    def __init__(self, op, structure: TensorStructure, ncv: int = 3, reorthogonalize: bool = True) -> None:
-        if ncv < 1:
-            raise ValueError("ncv must be at least 1")
+    def __init__(self, op, structure: TensorStructure, ncv: int = 3, reorthogonalize: bool = True,
+                 tol: float = 1e-13) -> None:
+        from tenet._native import TRIDIAG_MAX
+
+        if not 1 <= ncv <= TRIDIAG_MAX:
+            raise ValueError(f"ncv must lie in [1, {TRIDIAG_MAX}]")
         dev = op.fl.device
-        self.op, self.structure, self.ncv = op, structure, ncv
+        self.op, self.structure, self.ncv, self.tol = op, structure, ncv, tol
         self.reorth = reorthogonalize
         n = structure.size
         f64 = dict(dtype=torch.float64, device=dev)
@@ -81,6 +84,9 @@ class KrylovGraph:
         self.alphas = torch.zeros(ncv, **f64)
         self.betas = torch.zeros(ncv, **f64)
         self.coef = torch.zeros(ncv + 1, **f64)
+        self.g = torch.zeros(ncv, **f64)
+        self.value = torch.zeros((), **f64)
+        self.out = torch.zeros(n, **f64)
         self.tmp = torch.zeros((), **f64)
         self.work = torch.empty(1024, **f64)
         self.weights = ops._weights(structure, dev)
@@ -90,13 +96,14 @@ class KrylovGraph:
         self.start_(self.V[0])
         for j in range(ncv):
             self._iteration(j)
+        self._finish()
         torch.cuda.synchronize(dev)
         self.graphs = []
         pool = torch.cuda.graph_pool_handle()
-        for j in range(ncv):
+        for step in [*range(ncv), "finish"]:
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g, pool=pool):
-                self._iteration(j)
+                self._finish() if step == "finish" else self._iteration(step)
             self.graphs.append(g)
 
     def start_(self, v: torch.Tensor) -> None:
@@ -122,30 +129,32 @@ class KrylovGraph:
         torch.sqrt(self.tmp, out=self.betas[j])
         _axpby(1.0, self.betas[j], 1, w, 0.0, self.V[j + 1])
 
+    def _finish(self) -> None:
+        from tenet.network.dmrg import ritz
+
+        ritz(self.alphas, self.betas, [self.V[k] for k in range(self.ncv)], self.tol,
+             coef=self.g, value=self.value, out=self.out)
+        _dot(self.out, self.out, self.weights, self.work, self.tmp)
+        torch.sqrt(self.tmp, out=self.tmp)
+        _axpby(1.0, self.tmp, 1, self.out, 0.0, self.out)
+
     def replay(self) -> None:
         for g in self.graphs:
             g.replay()
 
-    def solve(self, v: SymmetricTensor, tol: float = 1e-13, refine=None) -> tuple[float, SymmetricTensor]:
-        """The Lanczos step of ``tenet.network.lanczos`` from start vector ``v``."""
+    def solve(self, v: SymmetricTensor, refine=None) -> tuple[float, SymmetricTensor]:
+        """The Lanczos step of ``tenet.network.lanczos`` from start vector ``v``: the
+        replayed graphs leave the Ritz pair on the device; ``refine`` (an FP64 matvec)
+        closes it with ``residual_refine``.  The host reads the energy only."""
+        from tenet.network.dmrg import residual_refine
+
         if v.structure != self.structure:
             raise ValueError("KrylovGraph: the vector's structure differs from the captured one")
         x = v.data if v.dtype == torch.float64 else ops.astype(v, torch.float64).data
         self.start_(x)
         self.replay()
-        a_h = self.alphas.cpu().numpy()
-        b_h = self.betas.cpu().numpy()
-        m = self.ncv
-        for j in range(self.ncv - 1):
-            if b_h[j] < tol:
-                m = j + 1
-                break
-        tri = np.diag(a_h[:m]) + np.diag(b_h[: m - 1], 1) + np.diag(b_h[: m - 1], -1)
-        values, states = np.linalg.eigh(tri)
-        g = torch.from_numpy(np.ascontiguousarray(states[:, 0])).to(self.V.device)
-        out = SymmetricTensor(self.structure, g @ self.V[:m])
-        out = ops.divide(out, ops.norm(out))
-        value = float(values[0])
+        out = SymmetricTensor(self.structure, self.out.clone())
+        value = self.value
         if refine is not None:
-            value = float(ops.inner(out, refine(out)))
-        return value, out
+            value, out, _ = residual_refine(refine, out)
+        return float(value), out
