@@ -88,6 +88,16 @@ int tn_gemm_grouped(int precision, int dtype, int trans_a, int trans_b, const vo
                     const int64_t* segs, const int32_t* tiles, int64_t ntiles, int bm, int bn,
                     int grid, void* stream);
 
+/* The same grouped product for skinny problems (the W contractions of the generic two-site
+ * H_eff, network/env.py:430-431: K and one output dimension of a few MPO/physical
+ * channels): rows != 0 -- every problem has N <= 8, one thread per output row; rows == 0
+ * -- every problem has M <= 8, one thread per output column.  Same problem and segment
+ * tables as tn_gemm_grouped; arithmetic in the operands' type.  Tile table: 2 int32 per
+ * tile: (problem, block of 256 output lines). */
+int tn_gemm_skinny(int dtype, int trans_a, int trans_b, int rows, const void* A, const void* B,
+                   void* C, void* W, const int64_t* problems, const int64_t* segs,
+                   const int32_t* tiles, int64_t ntiles, void* stream);
+
 /* out[0] = sum_i w_i x_i y_i (w may be NULL for unit weights), FP64 accumulation,
  * deterministic two-pass reduction; `work` must hold >= 1024 doubles. */
 int tn_dot(int dtype, const void* x, const void* y, const double* w, int64_t n,
@@ -95,7 +105,8 @@ int tn_dot(int dtype, const void* x, const void* y, const double* w, int64_t n,
 
 /* y = alpha * x + beta * y with alpha/beta read from device scalars when the pointer
  * form is used (alpha_dev/beta_dev non-NULL) -- no host synchronization. The scalar
- * used is  s_host * (s_dev ? *s_dev : 1) , optionally inverted when inv_* is set. */
+ * used is  s_host * (s_dev ? *s_dev : 1) , optionally inverted when inv_* is set.  A
+ * factor equal to 0 drops its term without reading the operand (so 0 * inf is 0). */
 int tn_axpby(int dtype, int64_t n, double alpha, const double* alpha_dev, int inv_alpha,
              const void* x, double beta, const double* beta_dev, int inv_beta, void* y,
              void* stream);

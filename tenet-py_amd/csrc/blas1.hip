@@ -84,8 +84,12 @@ __global__ __launch_bounds__(kThreads) void axpby_kernel(int64_t n, double a, co
   const double beta = scalar(b, bd, binv);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
+    // a zero factor contributes nothing and its operand is not read: a Lanczos basis
+    // vector past a happy breakdown (non-finite after the division by beta ~ 0) enters the
+    // Ritz recombination with coefficient exactly 0
     const double yv = beta == 0.0 ? 0.0 : beta * static_cast<double>(y[i]);
-    y[i] = static_cast<T>(alpha * static_cast<double>(x[i]) + yv);
+    const double xv = alpha == 0.0 ? 0.0 : alpha * static_cast<double>(x[i]);
+    y[i] = static_cast<T>(xv + yv);
   }
 }
 

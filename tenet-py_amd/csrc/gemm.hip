@@ -454,7 +454,92 @@ int launch_prec(int ta, int tb, int bm, int bn, const void* A, const void* B, vo
   return launch_tile<PREC, true, true>(bm, bn, A, B, C, W, probs, segs, tiles, ntiles, grid, s);
 }
 
+// ---- skinny problems: one output line per thread ----------------------------------------
+// A problem whose output has at most kSkinnyMax columns (ROWS) or rows (!ROWS) and a short
+// K is pure data movement: one thread owns one output row (column), keeps its <= 8
+// accumulators in registers and streams the K values of its A row (B column); the small
+// operand is uniform across the workgroup and comes through the scalar cache.  Tile table:
+// 2 int32 per tile: (problem, block of 256 lines).
+constexpr int kSkinnyMax = 8;
+
+template <typename T, bool TA, bool TB, bool ROWS>
+__global__ __launch_bounds__(kThreads) void gemm_skinny_kernel(
+    const T* __restrict__ A, const T* __restrict__ B, T* __restrict__ C, T* __restrict__ W,
+    const int64_t* __restrict__ probs, const int64_t* __restrict__ segs, const int32_t* __restrict__ tiles) {
+  const int p = tiles[2 * blockIdx.x];
+  const int64_t line = static_cast<int64_t>(tiles[2 * blockIdx.x + 1]) * kThreads + threadIdx.x;
+  const int64_t* pw = probs + static_cast<int64_t>(p) * TN_PROB_WORDS;
+  const int64_t M = pw[0], N = pw[1], c_off = pw[2], ldc = pw[3];
+  const int64_t nl = ROWS ? M : N;   // lines: output rows (ROWS) or columns
+  const int nw = static_cast<int>(ROWS ? N : M);  // width of a line, <= kSkinnyMax
+  if (line >= nl || nw > kSkinnyMax) return;
+  T acc[kSkinnyMax];
+#pragma unroll
+  for (int j = 0; j < kSkinnyMax; ++j) acc[j] = T(0);
+  for (int64_t s = pw[4]; s < pw[5]; ++s) {
+    const int64_t* sw = segs + s * TN_SEG_WORDS;
+    const T* a = A + sw[0];
+    const T* b = B + sw[1];
+    const int64_t K = sw[2], lda = sw[3], ldb = sw[4];
+    for (int64_t k = 0; k < K; ++k) {
+      if (ROWS) {  // line = row m; j runs over the <= 8 columns
+        const T av = TA ? a[k * lda + line] : a[line * lda + k];
+#pragma unroll
+        for (int j = 0; j < kSkinnyMax; ++j)
+          if (j < nw) acc[j] += av * (TB ? b[j * ldb + k] : b[k * ldb + j]);
+      } else {  // line = column n; j runs over the <= 8 rows
+        const T bv = TB ? b[line * ldb + k] : b[k * ldb + line];
+#pragma unroll
+        for (int j = 0; j < kSkinnyMax; ++j)
+          if (j < nw) acc[j] += (TA ? a[k * lda + j] : a[j * lda + k]) * bv;
+      }
+    }
+  }
+  T* __restrict__ out = pw[6] ? W : C;
+#pragma unroll
+  for (int j = 0; j < kSkinnyMax; ++j)
+    if (j < nw) out[c_off + (ROWS ? line * ldc + j : j * ldc + line)] = acc[j];
+}
+
+template <typename T, bool ROWS>
+int launch_skinny(int ta, int tb, const void* A, const void* B, void* C, void* W, const int64_t* probs,
+                  const int64_t* segs, const int32_t* tiles, int ntiles, hipStream_t s) {
+  const T* a = static_cast<const T*>(A);
+  const T* b = static_cast<const T*>(B);
+  T* c = static_cast<T*>(C);
+  T* w = static_cast<T*>(W);
+#define TN_SKINNY_CASE(TA_, TB_)                                                                   \
+  if (ta == TA_ && tb == TB_) {                                                                    \
+    gemm_skinny_kernel<T, TA_, TB_, ROWS><<<ntiles, kThreads, 0, s>>>(a, b, c, w, probs, segs, tiles); \
+    return tn::check_launch("tn_gemm_skinny");                                                     \
+  }
+  TN_SKINNY_CASE(0, 0)
+  TN_SKINNY_CASE(1, 0)
+  TN_SKINNY_CASE(0, 1)
+  TN_SKINNY_CASE(1, 1)
+#undef TN_SKINNY_CASE
+  return tn::fail("tn_gemm_skinny: bad transpose flags");
+}
+
 }  // namespace
+
+extern "C" int tn_gemm_skinny(int dtype, int trans_a, int trans_b, int rows, const void* A,
+                              const void* B, void* C, void* W, const int64_t* problems,
+                              const int64_t* segs, const int32_t* tiles, int64_t ntiles,
+                              void* stream) {
+  if (ntiles <= 0) return 0;
+  if (ntiles > 0x7fffffffLL) return tn::fail("tn_gemm_skinny: too many tiles");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nt = static_cast<int>(ntiles);
+  const int ta = trans_a ? 1 : 0, tb = trans_b ? 1 : 0;
+  if (dtype == TN_F64)
+    return rows ? launch_skinny<double, true>(ta, tb, A, B, C, W, problems, segs, tiles, nt, s)
+                : launch_skinny<double, false>(ta, tb, A, B, C, W, problems, segs, tiles, nt, s);
+  if (dtype == TN_F32)
+    return rows ? launch_skinny<float, true>(ta, tb, A, B, C, W, problems, segs, tiles, nt, s)
+                : launch_skinny<float, false>(ta, tb, A, B, C, W, problems, segs, tiles, nt, s);
+  return tn::fail("tn_gemm_skinny: unknown dtype");
+}
 
 extern "C" int tn_gemm_grouped(int precision, int dtype, int trans_a, int trans_b,
                                const void* A, const void* B, void* C, void* W,

@@ -4,6 +4,7 @@ by hand-written gfx950 kernels behind the reference TeNeT-py API (``SymmetricTen
 """
 
 from tenet import linalg, symmetry
+from tenet.dense import from_dense, to_dense
 from tenet.engine import gemm_precision, set_gemm_precision
 from tenet.fusion_tree import FusionTree, coupled_sectors, fusion_trees
 from tenet.ops import (
@@ -24,6 +25,7 @@ from tenet.ops import (
     transpose,
 )
 from tenet.plans import map_layout
+from tenet.serialize import load, save
 from tenet.space import IN, OUT, GradedSpace, Leg, Side
 from tenet.structure import FusionBlockKey, TensorStructure
 from tenet.symmetry import CapabilityError, StructureChangingError
@@ -50,17 +52,21 @@ __all__ = [
     "coupled_sectors",
     "divide",
     "einsum",
+    "from_dense",
     "fusion_trees",
     "gemm_precision",
     "inner",
     "linalg",
+    "load",
     "map_layout",
     "multiply",
     "norm",
     "repartition",
+    "save",
     "set_gemm_precision",
     "subtract",
     "symmetry",
     "tensordot",
+    "to_dense",
     "transpose",
 ]

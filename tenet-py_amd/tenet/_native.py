@@ -37,6 +37,7 @@ _SIGNATURES = {
     "tn_device_cus": (_i32, [ctypes.POINTER(_i32)]),
     "tn_recouple": (_i32, [_i32, _p, _p, _p, _p, _p, _p, _i64, _p]),
     "tn_gemm_grouped": (_i32, [_i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _p]),
+    "tn_gemm_skinny": (_i32, [_i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _i64, _p]),
     "tn_dot": (_i32, [_i32, _p, _p, _p, _i64, _p, _p, _p]),
     "tn_axpby": (_i32, [_i32, _i64, _d, _p, _i32, _p, _d, _p, _i32, _p, _p]),
     "tn_scale": (_i32, [_i32, _i64, _d, _p, _i32, _p, _p]),

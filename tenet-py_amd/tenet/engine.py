@@ -328,6 +328,8 @@ class GemmProgram:
 
     SPLIT_TARGET_TILES = 1536
     SPLIT_MIN_K = 256
+    SKINNY_MAX_W = 8    # the kernel's register line width (kSkinnyMax)
+    SKINNY_MAX_K = 64
 
     def __init__(self, problems, a_size, b_size, c_size, trans_a=False, trans_b=False,
                  tile=None, split: bool = True) -> None:
@@ -347,6 +349,16 @@ class GemmProgram:
         Ks0 = [max(sum(sg[2] for sg in sl), 1) for *_, sl in problems]
         self.tile = tuple(tile) if tile else (
             choose_tile([p[0] for p in problems], [p[1] for p in problems], Ks0) if problems else (64, 64))
+        # skinny launch: every output at most SKINNY_MAX_W wide in one dimension and K short
+        # -- a streaming pass, one thread per output line, instead of mostly-empty MFMA tiles
+        self.skinny = None
+        if problems and max(Ks0) <= self.SKINNY_MAX_K:
+            if all(p[1] <= self.SKINNY_MAX_W for p in problems):
+                self.skinny = 1
+            elif all(p[0] <= self.SKINNY_MAX_W for p in problems):
+                self.skinny = 0
+        if self.skinny is not None:
+            split = False
         bm, bn = self.tile
         ntiles = sum(-(-p[0] // bm) * -(-p[1] // bn) for p in problems)
         factor = 1
@@ -403,7 +415,17 @@ class GemmProgram:
         return chunks
 
     def _tables(self, tile, device):
+        if self.skinny is not None:
+            tile = "skinny"
         d = self._by_tile.get(tile)
+        if d is None and tile == "skinny":
+            lines = self.Ms if self.skinny else self.Ns
+            tiles = np.asarray([(p, b) for p, n in enumerate(lines) for b in range(-(-n // 256))],
+                               dtype=np.int32).reshape(-1, 2)
+            d = _DeviceCache(probs=self._probs.reshape(-1), segs=self._segs.reshape(-1),
+                             tiles=tiles.reshape(-1))
+            d.ntiles = len(tiles)
+            self._by_tile[tile] = d
         if d is None:
             bm, bn = tile
             rows = []
@@ -450,6 +472,15 @@ class GemmProgram:
         if events is not None:
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
+        if self.skinny is not None:
+            _native.check(_native.lib().tn_gemm_skinny(
+                _dtype_code(C.dtype), int(self.trans_a), int(self.trans_b), self.skinny,
+                A.data_ptr(), B.data_ptr(), C.data_ptr(), W.data_ptr(), t["probs"].data_ptr(),
+                t["segs"].data_ptr(), t["tiles"].data_ptr(), ntiles, _native.stream_ptr()))
+            if events is not None:
+                e1.record()
+                events.append((e0, e1, self.flops))
+            return
         _native.check(
             _native.lib().tn_gemm_grouped(
                 _native.PRECISIONS[prec],

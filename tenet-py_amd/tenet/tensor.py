@@ -123,6 +123,21 @@ class SymmetricTensor:
         dev = torch.device(device) if device is not None else default_device()
         return cls(s, torch.from_numpy(flat).to(device=dev, dtype=_torch_dtype(dtype)))
 
+    @classmethod
+    def from_dense(cls, dense, legs: Sequence[Leg], *, atol: float | None = 1e-10, dtype=None,
+                   device=None) -> "SymmetricTensor":
+        """Project a dense array onto the symmetric subspace of ``legs`` (reference
+        ``tensor.py:1053``); refuses a non-symmetric array (``atol=None`` projects)."""
+        from tenet.dense import from_dense
+
+        return from_dense(dense, legs, atol=atol, dtype=_torch_dtype(dtype), device=device)
+
+    def to_dense(self) -> np.ndarray:
+        """Host float64 dense array: ``sum_tau A^(tau) (x) C^(tau)`` (reference ``tensor.py:1028``)."""
+        from tenet.dense import to_dense
+
+        return to_dense(self)
+
     # ---------------------------------------------------------------- views
     @property
     def legs(self) -> tuple:

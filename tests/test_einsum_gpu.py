@@ -128,3 +128,30 @@ def test_split_k_long_contraction(prec, tol):
     out = tenet.einsum("ab,bc->ac", a, b)
     tenet.set_gemm_precision("fp32")
     assert rel_err(to_oracle(out).flat(), ref.flat()) < tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+@pytest.mark.parametrize("shape", ["rows", "cols"])
+def test_skinny_contractions(prec, tol, shape):
+    """Outputs at most 8 wide in one dimension with a short K run as one streaming launch
+    (one thread per output line) instead of MFMA tiles."""
+    import tenet
+    from tenet.engine import contraction_exec
+
+    small, big = {0: 2, 1: 1, -1: 3}, {0: 900, 1: 700, -1: 500}
+    if shape == "rows":
+        la, lb = (big, small), (small, small)
+    else:
+        la, lb = (small, small), (small, big)
+    a = tenet.SymmetricTensor.random((leg(la[0], 1), leg(la[1], 0)), seed=31, device="cuda")
+    b = tenet.SymmetricTensor.random((leg(lb[0], 1), leg(lb[1], 0)), seed=32, device="cuda")
+    oa = O.OTensor.random((oleg(la[0], 1), oleg(la[1], 0)), 31)
+    ob = O.OTensor.random((oleg(lb[0], 1), oleg(lb[1], 0)), 32)
+    ex = contraction_exec(a.structure, b.structure, ((1,), (0,)), (0, 1))
+    assert ex.gemm.skinny == (1 if shape == "rows" else 0)
+    ref = O.einsum("ab,bc->ac", oa, ob)
+    if prec != "fp64":
+        a, b = a.to(dtype=torch.float32), b.to(dtype=torch.float32)
+    out = tenet.einsum("ab,bc->ac", a, b)
+    assert rel_err(to_oracle(out).flat(), ref.flat()) < tol
