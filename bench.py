@@ -336,10 +336,12 @@ def bench_su2(args) -> int:
     torch.cuda.synchronize()
     kern = {}
     for k, v in ev.items():
+        if not v:
+            continue
         ms = sum(e0.elapsed_time(e1) for e0, e1, _ in v) / len(v)
         kern[str(k)] = {"avg_ms": ms, "work": v[0][2]}
     rec = {k: x for k, x in kern.items() if k.startswith("recouple")}
-    gem = {k: x for k, x in kern.items() if not k.startswith("recouple")}
+    gem = {k: x for k, x in kern.items() if not k.startswith(("recouple", "reduce"))}
     rec_bytes = sum(x["work"] for x in rec.values())
     rec_ms = sum(x["avg_ms"] for x in rec.values())
     gemm_flops = sum(x["work"] for x in gem.values())

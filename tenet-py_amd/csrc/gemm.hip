@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(
   constexpr int SM = WM / 16, SN = WN / 16;
   constexpr int LA = BM * BK / kThreads;  // prefetch registers per thread
   constexpr int LB = BN * BK / kThreads;
-  __shared__ Smem<PREC, BM, BN> sm;
+  __shared__ __attribute__((aligned(16))) Smem<PREC, BM, BN> sm;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(
           *reinterpret_cast<u4*>(&sm.ah[buf][am[r]][akq[r]]) = h;
           *reinterpret_cast<u4*>(&sm.al[buf][am[r]][akq[r]]) = l;
         } else {
-          *reinterpret_cast<V4*>(&sm.a[buf][am[r]][akq[r]]) = ra[r];
+          *static_cast<V4*>(__builtin_assume_aligned(&sm.a[buf][am[r]][akq[r]], 16)) = ra[r];
         }
       }
 #pragma unroll
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(
           *reinterpret_cast<u4*>(&sm.bh[buf][bn_[r]][bkq[r]]) = h;
           *reinterpret_cast<u4*>(&sm.bl[buf][bn_[r]][bkq[r]]) = l;
         } else {
-          *reinterpret_cast<V4*>(&sm.b[buf][bn_[r]][bkq[r]]) = rb[r];
+          *static_cast<V4*>(__builtin_assume_aligned(&sm.b[buf][bn_[r]][bkq[r]], 16)) = rb[r];
         }
       }
     };

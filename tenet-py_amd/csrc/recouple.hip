@@ -50,24 +50,41 @@ __global__ __launch_bounds__(kThreads) void recouple_kernel(const T* __restrict_
   const int tid = threadIdx.x;
 
   if (mode < 0) {
-    // linear: element e of the job in destination C order
+    // linear: element e of the job in destination C order.  All of a thread's elements
+    // are decoded first and their loads issued together (independent, one round of
+    // memory latency per term instead of one per element).
     const int64_t base = tile * kLinTile;
-#pragma unroll 2
+    int64_t so[kLinPerThread], dof[kLinPerThread];
+    bool live[kLinPerThread];
+#pragma unroll
     for (int r = 0; r < kLinPerThread; ++r) {
-      int64_t e = base + r * kThreads + tid;
-      if (e >= nelem) break;
-      int64_t so = 0, dof = 0;
-      uint32_t rem = static_cast<uint32_t>(e);
+      const int64_t e = base + r * kThreads + tid;
+      live[r] = e < nelem;
+      so[r] = 0;
+      dof[r] = 0;
+      uint32_t rem = live[r] ? static_cast<uint32_t>(e) : 0u;
       for (int d = rank - 1; d >= 0; --d) {
         const uint32_t n = static_cast<uint32_t>(shape[d]);
         const uint32_t q = rem / n;
         const int64_t i = rem - q * n;
         rem = q;
-        so += i * sstr[d];
-        dof += i * dstr[d];
+        so[r] += i * sstr[d];
+        dof[r] += i * dstr[d];
       }
-      dst[dst_off + dof] = gather_terms(src, tsrc, tco, tb, te, so);
     }
+    T acc[kLinPerThread];
+#pragma unroll
+    for (int r = 0; r < kLinPerThread; ++r) acc[r] = T(0);
+    for (int64_t t = tb; t < te; ++t) {
+      const T c = static_cast<T>(tco[t]);
+      const T* __restrict__ s0 = src + tsrc[t];
+#pragma unroll
+      for (int r = 0; r < kLinPerThread; ++r)
+        if (live[r]) acc[r] += c * s0[so[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < kLinPerThread; ++r)
+      if (live[r]) dst[dst_off + dof[r]] = acc[r];
     return;
   }
 

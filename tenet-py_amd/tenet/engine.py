@@ -608,12 +608,23 @@ class ContractionExec:
         self.flops = self.gemm.flops
         _ = n_out_b
 
+    def prepare_b(self, b_data: torch.Tensor) -> torch.Tensor:
+        """``b`` in the layout the GEMM reads (its recouple applied, if it needs one): for an
+        operand that stays fixed across many runs (an environment, an MPO tensor), do this
+        once and pass the result with ``b_ready=True``."""
+        if self.b.program is None:
+            return b_data
+        bp = torch.empty(self.b.structure.size, dtype=b_data.dtype, device=b_data.device)
+        self.b.program.run(b_data, bp)
+        return bp
+
     def run(self, a_data: torch.Tensor, b_data: torch.Tensor, precision=None, tile=None,
             out: torch.Tensor | None = None, events: list | None = None,
-            reduce_events: list | None = None) -> torch.Tensor:
+            reduce_events: list | None = None, b_ready: bool = False) -> torch.Tensor:
         """Run the contraction; when ``events`` is a list, the GEMM kernel launch is
         bracketed by two recorded events appended as ``(start, end, flops)`` (and a split-K
-        reduction, if any, into ``reduce_events``)."""
+        reduction, if any, into ``reduce_events``).  ``b_ready``: ``b_data`` already went
+        through ``prepare_b``."""
         dtype, dev = a_data.dtype, a_data.device
         if b_data.dtype != dtype:
             raise TypeError("tensordot: operands have different dtypes")
@@ -622,11 +633,7 @@ class ContractionExec:
             self.a.program.run(a_data, ap)
         else:
             ap = a_data
-        if self.b.program is not None:
-            bp = torch.empty(self.b.structure.size, dtype=dtype, device=dev)
-            self.b.program.run(b_data, bp)
-        else:
-            bp = b_data
+        bp = b_data if b_ready else self.prepare_b(b_data)
         if out is None:
             out = torch.empty(self.out_structure.size, dtype=dtype, device=dev)
         target = out if self.c_program is None else torch.empty(

@@ -6,9 +6,10 @@ path (reference ``network/env.py:416-432``, with the ``_composed`` bends of
 recouple program or nothing (U(1) bends are pure block re-indexings and move no data),
 plus one compiled block-sparse GEMM program.  Static operands (environments and MPO
 tensors) are bent and, for the FP32/BF16x3 arms, cast to FP32 once at construction; the
-Lanczos vector stays FP64 and is cast on entry/exit.  Every launch of one apply is a
-fixed sequence on the current stream, so the whole matvec can be captured as a hipGraph
-(``Heff2.graph``).
+Lanczos vector stays FP64 and is cast on entry/exit.  On the generic path the static
+operands are also recoupled once into the layouts the step GEMMs read.  Every launch of
+one apply is a fixed sequence on the current stream, so the whole matvec (and a whole
+Lanczos iteration around it) can be captured as a hipGraph (``tenet.network.graph``).
 """
 
 from __future__ import annotations
@@ -78,7 +79,12 @@ class _Chain:
             raise ValueError("heff2: the output structure differs from the input's")
         self.s_out = s
 
-    def run(self, x: torch.Tensor, datas, precision: str, events: dict | None = None) -> torch.Tensor:
+    def prepare(self, datas) -> tuple:
+        """The static operands in the layouts the step GEMMs read (done once per operator)."""
+        return tuple(ex.prepare_b(d) for (_, _, ex), d in zip(self.steps, datas))
+
+    def run(self, x: torch.Tensor, datas, precision: str, events: dict | None = None,
+            prepared: bool = False) -> torch.Tensor:
         cur = x
         for k, ((rep, s, ex), data) in enumerate(zip(self.steps, datas)):
             if rep is not None:
@@ -93,7 +99,7 @@ class _Chain:
                 cur = nxt
             ev = None if events is None else events.setdefault(k, [])
             rev = None if events is None else events.setdefault(f"reduce{k}", [])
-            cur = ex.run(cur, data, precision, events=ev, reduce_events=rev)
+            cur = ex.run(cur, data, precision, events=ev, reduce_events=rev, b_ready=prepared)
         return cur
 
 
@@ -307,6 +313,7 @@ class Heff2:
         dt = torch.float64 if precision == "fp64" else torch.float32
         self.dtype = dt
         self._chains: dict = {}
+        self._prepared: dict = {}
         self.fused = bool(getattr(fl.provider, "bosonic", False)) if fused is None else fused
         if self.fused:
             self._fr = ops.astype(fr, dt)
@@ -344,11 +351,19 @@ class Heff2:
         """``H_eff x`` on a flat buffer of ``structure``; returns a buffer of x's dtype.
         ``events`` (a dict) collects per-step GEMM events for kernel timing."""
         c = self.chain(structure)
+        if self.fused:
+            datas, kw = self._datas, {}
+        else:
+            # static operands recoupled once into the layouts the step GEMMs read
+            datas = self._prepared.get(structure)
+            if datas is None:
+                datas = self._prepared[structure] = c.prepare(self._datas)
+            kw = {"prepared": True}
         if x.dtype != self.dtype:
-            y = c.run(ops.astype(SymmetricTensor(structure, x), self.dtype).data, self._datas,
-                      self.precision, events)
+            y = c.run(ops.astype(SymmetricTensor(structure, x), self.dtype).data, datas,
+                      self.precision, events, **kw)
             return ops.astype(SymmetricTensor(c.s_out, y), x.dtype).data
-        return c.run(x, self._datas, self.precision, events)
+        return c.run(x, datas, self.precision, events, **kw)
 
     def __call__(self, aa: SymmetricTensor) -> SymmetricTensor:
         return SymmetricTensor(aa.structure, self.apply_data(aa.structure, aa.data))
