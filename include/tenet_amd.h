@@ -77,6 +77,12 @@ int tn_device_cus(int* out);
 int tn_recouple(int dtype, const void* src, void* dst, const int64_t* jobs,
                 const int64_t* term_src, const double* term_coeff, const int64_t* tiles,
                 int64_t ntiles, void* stream);
+/* The same with a type conversion on the way (src_dtype -> dst_dtype, arithmetic in the
+ * wider type): the precision staging of an FP32 H_eff (ar.do("astype") on the blocks)
+ * fused with the block permutation into the layout its GEMMs read. */
+int tn_recouple_cast(int src_dtype, int dst_dtype, const void* src, void* dst,
+                     const int64_t* jobs, const int64_t* term_src, const double* term_coeff,
+                     const int64_t* tiles, int64_t ntiles, void* stream);
 
 /* For every problem p: C_p = sum_s op(A_s) op(B_s) with op = transpose when trans_a/b,
  * written to C, or to the workspace W when the problem is flagged as a split-K partial.

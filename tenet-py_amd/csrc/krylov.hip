@@ -5,12 +5,12 @@
 // j+1), truncates at the first beta below `tol` (the happy breakdown of the recurrence:
 // the Krylov space is invariant there, so the leading block holds the answer), and writes
 // the lowest eigenvalue and its unit eigenvector (zeros past the truncation, first nonzero
-// component positive).  The eigenvalue comes from bisection on the Sturm count (Givens'
-// method), converged to the spacing of doubles; the eigenvector from three steps of
-// inverse iteration on T - lambda I with a pivoted tridiagonal LU (the scheme of LAPACK's
-// dgttrf/dgtts2).  One workgroup; the sequential part runs on one lane with the matrix in
-// LDS.  The same routine is compiled for the host (tn_tridiag_ground_host), which is how
-// the CPU test suite checks it against LAPACK without a GPU.
+// component positive).  The eigenvalue comes from the Sturm count (Givens' method):
+// 64-way multisection across one wavefront on the device, plain bisection in the host twin
+// (tn_tridiag_ground_host, which is how the CPU test suite checks the routine against
+// LAPACK without a GPU), both converged to the spacing of doubles; the eigenvector from
+// three steps of inverse iteration on T - lambda I with a pivoted tridiagonal LU (the
+// scheme of LAPACK's dgttrf/dgtts2) on one lane, with the matrix in LDS.
 #include <math.h>
 
 #include "tn_common.hpp"
@@ -37,9 +37,16 @@ __host__ __device__ inline int sturm_below(const double* a, const double* b2, in
   return cnt;
 }
 
-// a[0..n), b[0..n-1): the matrix; writes *value and x[0..m); returns m.
-__host__ __device__ inline int tridiag_ground(const double* a, const double* b, int n, double tol,
-                                              double* value, double* x, TridiagWork& w) {
+struct TridiagSetup {
+  int m;                 // leading block kept (happy breakdown at the first |beta| < tol)
+  double lo, hi;         // Gershgorin interval, widened a little
+  double pivmin, tnorm;  // smallest pivot allowed; scale of the matrix
+};
+
+// a[0..n), b[0..n-1): the matrix.  Fills w.b2 and returns the truncation and interval.
+__host__ __device__ inline TridiagSetup tridiag_setup(const double* a, const double* b, int n, double tol,
+                                                      TridiagWork& w) {
+  TridiagSetup st{};
   int m = n;
   for (int j = 0; j + 1 < n; ++j)
     if (fabs(b[j]) < tol) {
@@ -47,31 +54,32 @@ __host__ __device__ inline int tridiag_ground(const double* a, const double* b, 
       break;
     }
   for (int i = 0; i + 1 < m; ++i) w.b2[i] = b[i] * b[i];
-  double lo = INFINITY, hi = -INFINITY;  // Gershgorin interval
+  double lo = INFINITY, hi = -INFINITY;
   for (int i = 0; i < m; ++i) {
     const double r = (i ? fabs(b[i - 1]) : 0.0) + (i + 1 < m ? fabs(b[i]) : 0.0);
     lo = fmin(lo, a[i] - r);
     hi = fmax(hi, a[i] + r);
   }
   const double span = fmax(hi - lo, 1e-300);
-  const double tnorm = fmax(fabs(lo), fabs(hi));
-  // smallest pivot magnitude allowed in the Sturm recurrence and the LU: a zero pivot
-  // (lambda exactly an eigenvalue of a leading block) is replaced by it
-  const double pivmin = fmax(1e-290, 2.2e-16 * fmax(fabs(lo), fabs(hi)));
-  lo -= 1e-15 * span;
-  hi += 1e-15 * span;
-  for (int it = 0; it < 2200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid <= lo || mid >= hi) break;  // lo and hi are adjacent doubles
-    // relative resolution of doubles, with a floor far below the eps * |T| accuracy that
-    // any eigenvalue of T carries
-    if (hi - lo <= fmax(2.2e-16 * fmax(fabs(lo), fabs(hi)), 2.2e-19 * tnorm)) break;
-    if (sturm_below(a, w.b2, m, mid, pivmin) >= 1)
-      hi = mid;
-    else
-      lo = mid;
-  }
-  const double lam = 0.5 * (lo + hi);
+  st.m = m;
+  st.tnorm = fmax(fabs(lo), fabs(hi));
+  // a zero pivot (lambda exactly an eigenvalue of a leading block) is replaced by pivmin
+  st.pivmin = fmax(1e-290, 2.2e-16 * st.tnorm);
+  st.lo = lo - 1e-15 * span;
+  st.hi = hi + 1e-15 * span;
+  return st;
+}
+
+// The bracket is resolved: the relative resolution of doubles, with a floor far below the
+// eps * |T| accuracy that any eigenvalue of T carries.
+__host__ __device__ inline bool bracket_done(double lo, double hi, double tnorm) {
+  return hi - lo <= fmax(2.2e-16 * fmax(fabs(lo), fabs(hi)), 2.2e-19 * tnorm);
+}
+
+// Unit eigenvector of T[:m,:m] at eigenvalue lam: three steps of inverse iteration on
+// T - lam I with the pivoted tridiagonal LU; first nonzero entry made positive.
+__host__ __device__ inline void tridiag_vector(const double* a, const double* b, int m, double lam,
+                                               double pivmin, double* x, TridiagWork& w) {
   double *dl = w.dl, *dd = w.dd, *du = w.du, *du2 = w.du2;
   int* piv = w.piv;
   for (int i = 0; i < m; ++i) {
@@ -130,26 +138,66 @@ __host__ __device__ inline int tridiag_ground(const double* a, const double* b, 
   while (first + 1 < m && x[first] == 0.0) ++first;
   if (x[first] < 0.0)
     for (int i = 0; i < m; ++i) x[i] = -x[i];
-  *value = lam;
-  return m;
 }
 
+// The whole routine on one thread (host twin): bisection on the Sturm count.
+inline int tridiag_ground_serial(const double* a, const double* b, int n, double tol, double* value,
+                                 double* x, TridiagWork& w) {
+  const TridiagSetup st = tridiag_setup(a, b, n, tol, w);
+  double lo = st.lo, hi = st.hi;
+  for (int it = 0; it < 2200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi || bracket_done(lo, hi, st.tnorm)) break;
+    if (sturm_below(a, w.b2, st.m, mid, st.pivmin) >= 1)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  const double lam = 0.5 * (lo + hi);
+  tridiag_vector(a, b, st.m, lam, st.pivmin, x, w);
+  *value = lam;
+  return st.m;
+}
+
+// Device: 64-way multisection -- every lane evaluates the Sturm count at one of 64 interior
+// points of the bracket, a ballot finds the first point with an eigenvalue below it, and the
+// bracket shrinks 65-fold per round (about nine rounds to the resolution of doubles).  The
+// vector then comes from lane 0.
 __global__ __launch_bounds__(64) void tridiag_ground_kernel(const double* __restrict__ alpha,
                                                             const double* __restrict__ beta, int n,
                                                             double tol, double* __restrict__ value,
                                                             double* __restrict__ coef) {
   __shared__ double a[kMax], b[kMax], x[kMax];
   __shared__ TridiagWork w;
-  __shared__ int msh;
+  __shared__ TridiagSetup st;
   const int t = threadIdx.x;
   for (int i = t; i < n; i += 64) {
     a[i] = alpha[i];
     b[i] = i + 1 < n ? beta[i] : 0.0;
   }
   __syncthreads();
-  if (t == 0) msh = tridiag_ground(a, b, n, tol, value, x, w);
+  if (t == 0) st = tridiag_setup(a, b, n, tol, w);
   __syncthreads();
-  const int m = msh;
+  const int m = st.m;
+  double lo = st.lo, hi = st.hi;  // identical on every lane: every update below is uniform
+  for (int round = 0; round < 64; ++round) {
+    if (bracket_done(lo, hi, st.tnorm)) break;
+    const double step = (hi - lo) / 65.0;
+    if (lo + step <= lo) break;  // the bracket spans a few doubles
+    const bool below = sturm_below(a, w.b2, m, lo + (t + 1) * step, st.pivmin) >= 1;
+    const unsigned long long mask = __ballot(below);
+    const int k = mask ? __ffsll(mask) - 1 : 64;  // first of the 64 points with a count >= 1
+    const double nlo = k > 0 ? lo + k * step : lo;
+    const double nhi = k < 64 ? lo + (k + 1) * step : hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  if (t == 0) {
+    const double lam = 0.5 * (lo + hi);
+    tridiag_vector(a, b, m, lam, st.pivmin, x, w);
+    value[0] = lam;
+  }
+  __syncthreads();
   for (int i = t; i < n; i += 64) coef[i] = i < m ? x[i] : 0.0;
 }
 
@@ -169,7 +217,7 @@ extern "C" int tn_tridiag_ground_host(const double* alpha, const double* beta, i
   double b[kMax], x[kMax];
   TridiagWork w;
   for (int i = 0; i < n; ++i) b[i] = i + 1 < n ? beta[i] : 0.0;
-  const int m = tridiag_ground(alpha, b, n, tol, value, x, w);
+  const int m = tridiag_ground_serial(alpha, b, n, tol, value, x, w);
   for (int i = 0; i < n; ++i) coef[i] = i < m ? x[i] : 0.0;
   return 0;
 }

@@ -20,22 +20,25 @@ constexpr int kLinPerThread = 8;
 constexpr int kLinTile = kThreads * kLinPerThread;
 constexpr int kT = 64;  // LDS tile edge
 
-template <typename T>
-__device__ __forceinline__ T gather_terms(const T* __restrict__ src, const int64_t* __restrict__ tsrc,
+template <typename T, typename TS>
+__device__ __forceinline__ T gather_terms(const TS* __restrict__ src, const int64_t* __restrict__ tsrc,
                                           const double* __restrict__ tco, int64_t tb, int64_t te,
                                           int64_t soff) {
   T acc = T(0);
-  for (int64_t t = tb; t < te; ++t) acc += static_cast<T>(tco[t]) * src[tsrc[t] + soff];
+  for (int64_t t = tb; t < te; ++t) acc += static_cast<T>(tco[t]) * static_cast<T>(src[tsrc[t] + soff]);
   return acc;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void recouple_kernel(const T* __restrict__ src,
-                                                            T* __restrict__ dst,
+// TS: source element type, TD: destination element type (a cast rides along with the
+// permutation); T: the arithmetic type, the wider of the two.
+template <typename TS, typename TD>
+__global__ __launch_bounds__(kThreads) void recouple_kernel(const TS* __restrict__ src,
+                                                            TD* __restrict__ dst,
                                                             const int64_t* __restrict__ jobs,
                                                             const int64_t* __restrict__ tsrc,
                                                             const double* __restrict__ tco,
                                                             const int64_t* __restrict__ tiles) {
+  using T = decltype(TS(0) * TD(0));
   __shared__ T lds[kT][kT + 1];
   const int64_t job_id = tiles[2 * blockIdx.x];
   const int64_t tile = tiles[2 * blockIdx.x + 1];
@@ -77,14 +80,14 @@ __global__ __launch_bounds__(kThreads) void recouple_kernel(const T* __restrict_
     for (int r = 0; r < kLinPerThread; ++r) acc[r] = T(0);
     for (int64_t t = tb; t < te; ++t) {
       const T c = static_cast<T>(tco[t]);
-      const T* __restrict__ s0 = src + tsrc[t];
+      const TS* __restrict__ s0 = src + tsrc[t];
 #pragma unroll
       for (int r = 0; r < kLinPerThread; ++r)
-        if (live[r]) acc[r] += c * s0[so[r]];
+        if (live[r]) acc[r] += c * static_cast<T>(s0[so[r]]);
     }
 #pragma unroll
     for (int r = 0; r < kLinPerThread; ++r)
-      if (live[r]) dst[dst_off + dof[r]] = acc[r];
+      if (live[r]) dst[dst_off + dof[r]] = static_cast<TD>(acc[r]);
     return;
   }
 
@@ -115,36 +118,52 @@ __global__ __launch_bounds__(kThreads) void recouple_kernel(const T* __restrict_
     const int idx = r * kThreads + tid;
     const int j = idx % kT, i = idx / kT;
     if (i < ni && j < nj)
-      lds[i][j] = gather_terms(src, tsrc, tco, tb, te, sbase + (i0 + i) * sI + (j0 + j) * sJ);
+      lds[i][j] = gather_terms<T>(src, tsrc, tco, tb, te, sbase + (i0 + i) * sI + (j0 + j) * sJ);
   }
   __syncthreads();
   // store: threads run along I (destination-contiguous)
   for (int r = 0; r < kT * kT / kThreads; ++r) {
     const int idx = r * kThreads + tid;
     const int i = idx % kT, j = idx / kT;
-    if (i < ni && j < nj) dst[dst_off + dbase + (j0 + j) * dJ + (i0 + i) * dI] = lds[i][j];
+    if (i < ni && j < nj) dst[dst_off + dbase + (j0 + j) * dJ + (i0 + i) * dI] = static_cast<TD>(lds[i][j]);
   }
 }
 
 }  // namespace
 
-extern "C" int tn_recouple(int dtype, const void* src, void* dst, const int64_t* jobs,
-                           const int64_t* term_src, const double* term_coeff,
-                           const int64_t* tiles, int64_t ntiles, void* stream) {
+namespace {
+
+template <typename TS, typename TD>
+void launch_recouple(const void* src, void* dst, const int64_t* jobs, const int64_t* term_src,
+                     const double* term_coeff, const int64_t* tiles, int64_t ntiles, hipStream_t s) {
+  recouple_kernel<TS, TD><<<dim3(static_cast<unsigned>(ntiles)), dim3(kThreads), 0, s>>>(
+      static_cast<const TS*>(src), static_cast<TD*>(dst), jobs, term_src, term_coeff, tiles);
+}
+
+}  // namespace
+
+extern "C" int tn_recouple_cast(int src_dtype, int dst_dtype, const void* src, void* dst,
+                                const int64_t* jobs, const int64_t* term_src,
+                                const double* term_coeff, const int64_t* tiles, int64_t ntiles,
+                                void* stream) {
   if (ntiles <= 0) return 0;
   if (ntiles > 0x7fffffffLL) return tn::fail("tn_recouple: too many tiles");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  dim3 grid(static_cast<unsigned>(ntiles)), block(kThreads);
-  if (dtype == TN_F64) {
-    recouple_kernel<double><<<grid, block, 0, s>>>(static_cast<const double*>(src),
-                                                   static_cast<double*>(dst), jobs, term_src,
-                                                   term_coeff, tiles);
-  } else if (dtype == TN_F32) {
-    recouple_kernel<float><<<grid, block, 0, s>>>(static_cast<const float*>(src),
-                                                  static_cast<float*>(dst), jobs, term_src,
-                                                  term_coeff, tiles);
-  } else {
+  if (src_dtype == TN_F64 && dst_dtype == TN_F64)
+    launch_recouple<double, double>(src, dst, jobs, term_src, term_coeff, tiles, ntiles, s);
+  else if (src_dtype == TN_F32 && dst_dtype == TN_F32)
+    launch_recouple<float, float>(src, dst, jobs, term_src, term_coeff, tiles, ntiles, s);
+  else if (src_dtype == TN_F64 && dst_dtype == TN_F32)
+    launch_recouple<double, float>(src, dst, jobs, term_src, term_coeff, tiles, ntiles, s);
+  else if (src_dtype == TN_F32 && dst_dtype == TN_F64)
+    launch_recouple<float, double>(src, dst, jobs, term_src, term_coeff, tiles, ntiles, s);
+  else
     return tn::fail("tn_recouple: unknown dtype");
-  }
   return tn::check_launch("tn_recouple");
+}
+
+extern "C" int tn_recouple(int dtype, const void* src, void* dst, const int64_t* jobs,
+                           const int64_t* term_src, const double* term_coeff,
+                           const int64_t* tiles, int64_t ntiles, void* stream) {
+  return tn_recouple_cast(dtype, dtype, src, dst, jobs, term_src, term_coeff, tiles, ntiles, stream);
 }

@@ -36,6 +36,7 @@ _SIGNATURES = {
     "tn_version": (_i32, []),
     "tn_device_cus": (_i32, [ctypes.POINTER(_i32)]),
     "tn_recouple": (_i32, [_i32, _p, _p, _p, _p, _p, _p, _i64, _p]),
+    "tn_recouple_cast": (_i32, [_i32, _i32, _p, _p, _p, _p, _p, _p, _i64, _p]),
     "tn_gemm_grouped": (_i32, [_i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _p]),
     "tn_gemm_skinny": (_i32, [_i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _i64, _p]),
     "tn_dot": (_i32, [_i32, _p, _p, _p, _i64, _p, _p, _p]),

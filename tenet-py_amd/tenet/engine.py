@@ -184,15 +184,16 @@ class RecoupleProgram:
         return (self.moved_terms + self.elements) * itemsize
 
     def run(self, src: torch.Tensor, dst: torch.Tensor) -> None:
+        """One launch; when ``dst`` has another float type than ``src`` the conversion
+        rides along with the permutation."""
         if self.ntiles == 0:
             return
-        if src.dtype != dst.dtype:
-            raise TypeError("recouple: source and destination dtypes differ")
         if src.numel() < self.src_size or dst.numel() < self.dst_size:
             raise ValueError("recouple: buffers smaller than the compiled plan")
         t = self._tables.get(dst.device)
         _native.check(
-            _native.lib().tn_recouple(
+            _native.lib().tn_recouple_cast(
+                _dtype_code(src.dtype),
                 _dtype_code(dst.dtype),
                 src.data_ptr(),
                 dst.data_ptr(),
