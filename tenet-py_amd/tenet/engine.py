@@ -346,6 +346,10 @@ class GemmProgram:
                 b_hi = b_off + ((N - 1) * ldb + K if self.trans_b else (K - 1) * ldb + N)
                 if a_off < 0 or b_off < 0 or a_hi > a_size or b_hi > b_size:
                     raise ValueError("gemm segment reads outside its operand buffer")
+                # the kernel addresses a segment with 32-bit byte offsets from its base and
+                # marks rows outside a tile with offsets >= 2^31
+                if max(a_hi - a_off, b_hi - b_off) * 8 >= 2**31:
+                    raise ValueError("gemm segment spans 2 GiB or more of one operand")
                 self.flops += 2 * M * N * K
         Ks0 = [max(sum(sg[2] for sg in sl), 1) for *_, sl in problems]
         self.tile = tuple(tile) if tile else (
@@ -440,6 +444,11 @@ class GemmProgram:
             tiles = np.concatenate(rows) if rows else np.zeros((0, 4), np.int64)
             # longest K first; ties keep problem order
             tiles = tiles[np.argsort(-tiles[:, 3], kind="stable")]
+            if len(tiles) % 8 == 0:
+                # the kernel hands XCD x the table slice [x n/8, (x+1) n/8) (workgroup b
+                # runs on XCD b % 8); deal the sorted tiles round-robin into those slices
+                # so every XCD gets the same mix of long and short tiles, longest first
+                tiles = tiles[np.arange(len(tiles)).reshape(-1, 8).T.reshape(-1)]
             tiles[:, 3] = 0
             tiles = tiles.astype(np.int32)
             d = _DeviceCache(probs=self._probs.reshape(-1), segs=self._segs.reshape(-1),
