@@ -325,6 +325,11 @@ class GemmProgram:
     into K slices whose partial products go to a workspace; one multi-term recouple launch
     then sums the slices in a fixed order into C (deterministic split-K).  Tiles are
     issued longest-K first so the tail of the launch is short tiles.
+
+    ``gather=True``: every problem writes its slices (one when unsplit) to the workspace,
+    row-major with leading dimension N, and no reduction is compiled; ``slices[p]`` lists
+    the workspace offsets of problem p so the caller can fold the sum into a pass of its
+    own (the problems' ``c_off``/``ldc`` are ignored).
     """
 
     SPLIT_TARGET_TILES = 1536
@@ -333,13 +338,14 @@ class GemmProgram:
     SKINNY_MAX_K = 64
 
     def __init__(self, problems, a_size, b_size, c_size, trans_a=False, trans_b=False,
-                 tile=None, split: bool = True) -> None:
+                 tile=None, split: bool = True, gather: bool = False) -> None:
         self.trans_a, self.trans_b = bool(trans_a), bool(trans_b)
+        self.gather = bool(gather)
         self.a_size, self.b_size, self.c_size = a_size, b_size, c_size
         self.problems = problems  # unsplit, for merging into batched launches
         self.flops = 0
         for M, N, c_off, ldc, sl in problems:
-            if c_off < 0 or c_off + (M - 1) * ldc + N > c_size:
+            if not gather and (c_off < 0 or c_off + (M - 1) * ldc + N > c_size):
                 raise ValueError("gemm problem writes outside the output buffer")
             for a_off, b_off, K, lda, ldb in sl:
                 a_hi = a_off + ((K - 1) * lda + M if self.trans_a else (M - 1) * lda + K)
@@ -362,6 +368,8 @@ class GemmProgram:
                 self.skinny = 1
             elif all(p[0] <= self.SKINNY_MAX_W for p in problems):
                 self.skinny = 0
+        if self.skinny is not None and gather:
+            self.skinny = None  # the skinny kernel writes C directly
         if self.skinny is not None:
             split = False
         bm, bn = self.tile
@@ -370,23 +378,26 @@ class GemmProgram:
         if split and problems and ntiles < self.SPLIT_TARGET_TILES:
             factor = min(16, -(-self.SPLIT_TARGET_TILES // max(ntiles, 1)))
         rows, segs, red = [], [], []
+        self.slices = []
         ws = 0
         for (M, N, c_off, ldc, sl), kt in zip(problems, Ks0):
             nsplit = 1
             if factor > 1 and kt >= self.SPLIT_MIN_K:
                 nsplit = max(1, min(factor, kt // (self.SPLIT_MIN_K // 2)))
-            if nsplit == 1:
+            if nsplit == 1 and not gather:
                 rows.append((M, N, c_off, ldc, len(segs), len(segs) + len(sl), 0, kt))
                 segs.extend((*sg, 0) for sg in sl)
                 continue
-            chunks = self._split_segments(sl, kt, nsplit)
+            chunks = self._split_segments(sl, kt, nsplit) if nsplit > 1 else [list(sl)]
             offs = []
             for ch in chunks:
                 rows.append((M, N, ws, N, len(segs), len(segs) + len(ch), 1, sum(c[2] for c in ch)))
                 segs.extend((*c, 0) for c in ch)
                 offs.append(ws)
                 ws += M * N
-            red.append((c_off, (M, N), (ldc, 1), (N, 1), [(o, 1.0) for o in offs]))
+            self.slices.append(offs)
+            if not gather:
+                red.append((c_off, (M, N), (ldc, 1), (N, 1), [(o, 1.0) for o in offs]))
         self.ws_size = ws
         self.reduce = RecoupleProgram(red, ws, c_size) if red else None
         self._rows = rows
@@ -462,12 +473,13 @@ class GemmProgram:
         """One grouped launch (plus the split-K reduction when the program has one).  With
         ``events`` a list, the GEMM kernel alone is bracketed by two recorded events,
         appended as ``(start, end, flops)``; ``reduce_events`` does the same for the
-        reduction with its algorithmic bytes."""
+        reduction with its algorithmic bytes.  A gather program takes its workspace as C."""
         if self.nproblems == 0:
             return
         if not (A.dtype == B.dtype == C.dtype):
             raise TypeError("gemm: operand dtypes differ")
-        if A.numel() < self.a_size or B.numel() < self.b_size or C.numel() < self.c_size:
+        c_need = self.ws_size if self.gather else self.c_size
+        if A.numel() < self.a_size or B.numel() < self.b_size or C.numel() < c_need:
             raise ValueError("gemm: buffers smaller than the compiled plan")
         prec = precision or gemm_precision(C.dtype)
         tile = tuple(tile) if tile else self.tile
@@ -478,7 +490,10 @@ class GemmProgram:
             # than one workgroup per tile on uneven sector sizes (static assignment loses
             # the dispatcher's dynamic balancing), so it is opt-in.
             grid = min(ntiles, resident_workgroups(prec, tile))
-        W = torch.empty(self.ws_size, dtype=C.dtype, device=C.device) if self.ws_size else C
+        if self.gather:
+            W = C
+        else:
+            W = torch.empty(self.ws_size, dtype=C.dtype, device=C.device) if self.ws_size else C
         if events is not None:
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()

@@ -104,7 +104,7 @@ class BatchedHeff2:
     def __init__(self, instances, precision: str = "fp32", tiles: dict | None = None) -> None:
         dt = torch.float64 if precision == "fp64" else torch.float32
         self.precision, self.dtype = precision, dt
-        self.ops = [Heff2(fl, w1, w2, fr, precision) for fl, w1, w2, fr, _ in instances]
+        self.ops = [Heff2(fl, w1, w2, fr, precision, layout="blocks") for fl, w1, w2, fr, _ in instances]
         self.structures = [inst[4].structure for inst in instances]
         chains = [op.chain(s) for op, s in zip(self.ops, self.structures)]
         for ch in chains:

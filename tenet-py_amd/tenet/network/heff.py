@@ -300,13 +300,274 @@ class _FusedChain:
         return out
 
 
+def _mpo_jobs(inputs, outputs, w12, src_size, dst_size):
+    """``tn_mpo_apply`` tables from per-(a, s) channel lists.  ``inputs[(a, s)]`` and
+    ``outputs[(a, s)]`` hold ``(offset, row_stride, dense_index, R, S)`` per channel slab;
+    the coefficient of output o and input i is ``w12[o.dense + i.dense]``."""
+    jobs, ins, outs, coef, tiles = [], [], [], [], []
+    for key, olist in outputs.items():
+        ilist = inputs.get(key, [])
+        if len(ilist) > 32:
+            raise ValueError("two-site MPO slab pair has more than 32 input channels")
+        R, S = olist[0][3], olist[0][4]
+        cm = np.zeros((len(olist), 32), dtype=np.float64)
+        for oi, o in enumerate(olist):
+            for ki, i in enumerate(ilist):
+                cm[oi, ki] = w12[o[2] + i[2]]
+        j = len(jobs)
+        jobs.append((R, S, len(ins), len(ins) + len(ilist), len(outs), len(outs) + len(olist), len(coef), 0))
+        ins.extend((i[0], i[1]) for i in ilist)
+        outs.extend((o[0], o[1]) for o in olist)
+        coef.extend(cm.reshape(-1).tolist())
+        nt = -(-(R * S) // 1024)
+        tiles.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
+        for off, rs, _, r_, s_ in ilist:
+            if r_ != R or s_ != S or off < 0 or off + (R - 1) * rs + S > src_size:
+                raise ValueError("MPO pass reads outside its source")
+        for off, rs, _, _, _ in olist:
+            if off < 0 or off + (R - 1) * rs + S > dst_size:
+                raise ValueError("MPO pass writes outside its destination")
+    return jobs, ins, outs, coef, tiles
+
+
+class _SectorChain:
+    """Abelian fast path on charge-sector stacked operands.
+
+    The three steps of ``_FusedChain`` with every block of one charge sector merged into one
+    matrix, so each GEMM is one problem per sector instead of one per block pair (far less
+    tile padding on uneven sector sizes):
+
+    0. entry: the blocks of ``aa`` regrouped by their right sector r (whole blocks, rows of
+       length d_r) and, for the FP32 arithmetics, cast on the way.
+    1. ``t1_r = x_r . FR_r`` per r: rows (a, p, q), columns (y, s) over every FR block of
+       sector r (``FR_r`` stacked once per operator).
+    2. the fused two-site MPO pass, writing ``t3_B`` per left sector B of the output: rows
+       (a, x) in the order of the stacked ``FL_B``, columns (P, Q, s) over the output blocks.
+    3. ``out_B = FL_B^T . t3_B`` per B into a workspace (split into K slices when the grid
+       would be too small), then one exit pass sums the slices, scatters every output block
+       into ``aa``'s layout and casts to the caller's type.
+    """
+
+    def __init__(self, s_in: TensorStructure, fr_s: TensorStructure, fl_s: TensorStructure,
+                 w_left, w_right) -> None:
+        from tenet.engine import GemmProgram, RecoupleProgram, _DeviceCache
+
+        self.s_out = s_in
+        w1, o1 = w_left[0], w_left[1]    # (x, P, p, m)
+        w2, o2 = w_right[0], w_right[1]  # (m, Q, q, y)
+        w12 = np.einsum("xPpm,mQqy->xPQpqy", w1, w2)
+        L = s_in.layout
+        # ---- entry: aa blocks grouped by r ----
+        rows_r: dict = {}
+        for i, key in enumerate(s_in.block_order):
+            a, p, q, r = s_in.axis_sectors(key)
+            rows_r.setdefault(r, []).append(i)
+        # ---- FR stacked per r: d_r x N_r, columns (y, s) blocks ----
+        cols_r: dict = {}
+        for i, key in enumerate(fr_s.block_order):
+            r, y, sg = fr_s.axis_sectors(key)
+            cols_r.setdefault(r, []).append(i)
+        xs_base, x_rowoff, xs_regions, acc = {}, {}, [], 0
+        frs_base, fr_coloff, n_r, frs_regions, facc = {}, {}, {}, [], 0
+        t1_base, t1_acc = {}, 0
+        problems1 = []
+        for r, blocks in rows_r.items():
+            dr = int(L.shapes[blocks[0]][3])
+            xs_base[r] = acc
+            m = 0
+            for i in blocks:
+                da, dp, dq, _ = (int(v) for v in L.shapes[i])
+                x_rowoff[i] = m
+                n = da * dp * dq * dr
+                xs_regions.append((acc + m * dr, (n,), (1,), (1,), [(int(L.offsets[i]), 1.0)]))
+                m += da * dp * dq
+            acc += m * dr
+            fcols = cols_r.get(r, [])
+            if not fcols:
+                continue
+            if r not in frs_base:
+                frs_base[r] = facc
+                c0 = 0
+                for i in fcols:
+                    fdr, my, ds = (int(v) for v in fr_s.layout.shapes[i])
+                    fr_coloff[i] = c0
+                    c0 += my * ds
+                n_r[r] = c0
+                for i in fcols:
+                    fdr, my, ds = (int(v) for v in fr_s.layout.shapes[i])
+                    frs_regions.append((facc + fr_coloff[i], (fdr, my * ds), (c0, 1), (my * ds, 1),
+                                        [(int(fr_s.layout.offsets[i]), 1.0)]))
+                facc += dr * c0
+            t1_base[r] = t1_acc
+            problems1.append((m, n_r[r], t1_acc, n_r[r], [(xs_base[r], frs_base[r], dr, dr, n_r[r])]))
+            t1_acc += m * n_r[r]
+        self.xs_size, self.frs_size, self.t1_size = acc, facc, t1_acc
+        self.entry = RecoupleProgram(xs_regions, s_in.size, max(acc, 1))
+        self.fr_stack = RecoupleProgram(frs_regions, fr_s.size, max(facc, 1))
+        # 64 x 64 workgroup tiles: the stacked sectors are few and uneven, and larger tiles
+        # leave too few workgroups for the device (measured on the D = 2048 U(1) bond)
+        self.gemm1 = GemmProgram(problems1, max(acc, 1), max(facc, 1), max(t1_acc, 1), tile=(64, 64))
+        # ---- FL stacked per B: K_B x d_B, rows (a, x) blocks; output blocks per B ----
+        fl_rows: dict = {}
+        for i, key in enumerate(fl_s.block_order):
+            a, x, B = fl_s.axis_sectors(key)
+            fl_rows.setdefault(B, []).append(i)
+        out_cols: dict = {}
+        for j, key in enumerate(s_in.block_order):
+            B, P, Q, sg = s_in.axis_sectors(key)
+            out_cols.setdefault(B, []).append(j)
+        fls_base, fl_rowoff, k_b, fls_regions, lacc = {}, {}, {}, [], 0
+        t3_base, out_coloff, n_b, t3_acc = {}, {}, {}, 0
+        problems3, order3 = [], []
+        for B, outs_j in out_cols.items():
+            db = int(L.shapes[outs_j[0]][0])
+            c0 = 0
+            for j in outs_j:
+                _, dP, dQ, ds = (int(v) for v in L.shapes[j])
+                out_coloff[j] = c0
+                c0 += dP * dQ * ds
+            n_b[B] = c0
+            frows = fl_rows.get(B, [])
+            kb = 0
+            fls_base[B] = lacc
+            for i in frows:
+                da, mx, fdb = (int(v) for v in fl_s.layout.shapes[i])
+                fl_rowoff[i] = kb
+                fls_regions.append((lacc + kb * fdb, (da * mx * fdb,), (1,), (1,),
+                                    [(int(fl_s.layout.offsets[i]), 1.0)]))
+                kb += da * mx
+            k_b[B] = kb
+            lacc += kb * db
+            t3_base[B] = t3_acc
+            t3_acc += kb * c0
+            problems3.append((db, c0, 0, c0, [(fls_base[B], t3_base[B], kb, db, c0)] if kb else []))
+            order3.append(B)
+        self.fls_size, self.t3_size = lacc, t3_acc
+        self.fl_stack = RecoupleProgram(fls_regions, fl_s.size, max(lacc, 1))
+        self.gemm4 = GemmProgram(problems3, max(lacc, 1), max(t3_acc, 1), 1, trans_a=True, tile=(64, 64),
+                                 gather=True)
+        # ---- MPO pass: t1 (r-stacked) -> t3 (B-stacked), one job per (a, s) ----
+        fr_by_r_s: dict = {}
+        for i in [i for ids in cols_r.values() for i in ids]:
+            r, y, sg = fr_s.axis_sectors(fr_s.block_order[i])
+            fr_by_r_s.setdefault(r, []).append((i, y, sg))
+        inputs: dict = {}
+        for i, key in enumerate(s_in.block_order):
+            a, p, q, r = s_in.axis_sectors(key)
+            if r not in t1_base:
+                continue
+            da, dp, dq, _ = (int(v) for v in L.shapes[i])
+            N = n_r[r]
+            for fi, y, sg in fr_by_r_s.get(r, ()):
+                _, my, ds = (int(v) for v in fr_s.layout.shapes[fi])
+                for ip in range(dp):
+                    for iq in range(dq):
+                        for iy in range(my):
+                            off = (t1_base[r] + (x_rowoff[i] + ip * dq + iq) * N + fr_coloff[fi] + iy * ds)
+                            dense = (o1[2][p] + ip, o2[2][q] + iq, o2[3][y] + iy)
+                            inputs.setdefault((a, sg), []).append((off, dp * dq * N, dense, da, ds))
+        outputs: dict = {}
+        for B in order3:
+            N = n_b[B]
+            for fi in fl_rows.get(B, ()):
+                a, x, _ = fl_s.axis_sectors(fl_s.block_order[fi])
+                da, mx, _ = (int(v) for v in fl_s.layout.shapes[fi])
+                for j in out_cols[B]:
+                    _, P, Q, sg = s_in.axis_sectors(s_in.block_order[j])
+                    _, dP, dQ, ds = (int(v) for v in L.shapes[j])
+                    for ix in range(mx):
+                        for iP in range(dP):
+                            for iQ in range(dQ):
+                                off = (t3_base[B] + (fl_rowoff[fi] + ix) * N + out_coloff[j] + (iP * dQ + iQ) * ds)
+                                dense = (o1[0][x] + ix, o1[1][P] + iP, o2[1][Q] + iQ)
+                                outputs.setdefault((a, sg), []).append((off, mx * N, dense, da, ds))
+        jobs, ins, outs, coef, tiles = _mpo_jobs(inputs, outputs, w12, max(t1_acc, 1), max(t3_acc, 1))
+        self.mpo_njobs = len(jobs)
+        tiles = np.concatenate(tiles) if tiles else np.zeros((0, 2), np.int64)
+        self.mpo_ntiles = len(tiles)
+        self.mpo_tables = _DeviceCache(
+            jobs=np.asarray(jobs, dtype=np.int64).reshape(-1),
+            ins=np.asarray(ins, dtype=np.int64).reshape(-1),
+            outs=np.asarray(outs, dtype=np.int64).reshape(-1),
+            coef64=np.asarray(coef, dtype=np.float64),
+            coef32=np.asarray(coef, dtype=np.float32),
+            tiles=tiles.reshape(-1),
+        )
+        self.mpo_bytes_elems = t1_acc + t3_acc
+        # ---- exit: sum of the K slices of out_B, scattered into aa's blocks ----
+        ex_regions = []
+        for B, sl in zip(order3, self.gemm4.slices):
+            N = n_b[B]
+            for j in out_cols[B]:
+                db, dP, dQ, ds = (int(v) for v in L.shapes[j])
+                n = dP * dQ * ds
+                ex_regions.append((int(L.offsets[j]), (db, n), (n, 1), (N, 1),
+                                   [(o + out_coloff[j], 1.0) for o in sl]))
+        self.exit = RecoupleProgram(ex_regions, max(self.gemm4.ws_size, 1), s_in.size)
+        self.flops = self.gemm1.flops + self.gemm4.flops
+
+    def prepare(self, datas) -> tuple:
+        """FR and FL stacked per sector (once per operator)."""
+        fr, fl = datas
+        frs = torch.empty(max(self.frs_size, 1), dtype=fr.dtype, device=fr.device)
+        fls = torch.empty(max(self.fls_size, 1), dtype=fl.dtype, device=fl.device)
+        self.fr_stack.run(fr, frs)
+        self.fl_stack.run(fl, fls)
+        return frs, fls
+
+    def run(self, x: torch.Tensor, datas, precision: str, events: dict | None = None,
+            dtype: torch.dtype | None = None) -> torch.Tensor:
+        """``H_eff x`` with the GEMMs in ``dtype`` (default: x's); returns x's type."""
+        from tenet import _native
+
+        frs, fls = datas
+        wt = dtype or x.dtype
+
+        def timed(name, fn, amount):
+            if events is None:
+                return fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            events.setdefault(name, []).append((e0, e1, amount))
+
+        xs = torch.empty(max(self.xs_size, 1), dtype=wt, device=x.device)
+        timed("entry", lambda: self.entry.run(x, xs), self.entry.bytes_moved(xs.element_size()))
+        t1 = torch.empty(max(self.t1_size, 1), dtype=wt, device=x.device)
+        self.gemm1.run(xs, frs, t1, precision,
+                       events=None if events is None else events.setdefault(0, []),
+                       reduce_events=None if events is None else events.setdefault("reduce0", []))
+        t3 = torch.empty(max(self.t3_size, 1), dtype=wt, device=x.device)
+        tb = self.mpo_tables.get(x.device)
+        _native.check(_native.lib().tn_mpo_apply(
+            _native.TN_F64 if wt == torch.float64 else _native.TN_F32,
+            t1.data_ptr(), t3.data_ptr(), tb["jobs"].data_ptr(), self.mpo_njobs,
+            tb["ins"].data_ptr(), tb["outs"].data_ptr(),
+            (tb["coef64"] if wt == torch.float64 else tb["coef32"]).data_ptr(),
+            tb["tiles"].data_ptr(), self.mpo_ntiles, _native.stream_ptr()))
+        ws = torch.empty(max(self.gemm4.ws_size, 1), dtype=wt, device=x.device)
+        self.gemm4.run(fls, t3, ws, precision,
+                       events=None if events is None else events.setdefault(3, []))
+        out = torch.empty(self.s_out.size, dtype=x.dtype, device=x.device)
+        timed("exit", lambda: self.exit.run(ws, out), self.exit.bytes_moved(ws.element_size()))
+        return out
+
+
 class Heff2:
     """Compiled ``H_eff`` of one bond at a fixed arithmetic (``fp64``, ``fp32``, ``bf16x3``)."""
 
     def __init__(self, fl: SymmetricTensor, w_left: SymmetricTensor, w_right: SymmetricTensor,
-                 fr: SymmetricTensor, precision: str = "fp64", fused: bool | None = None) -> None:
+                 fr: SymmetricTensor, precision: str = "fp64", fused: bool | None = None,
+                 layout: str = "sector") -> None:
+        """``layout`` (abelian symmetries): ``"sector"`` stacks every block of a charge
+        sector into one GEMM operand (``_SectorChain``); ``"blocks"`` keeps one GEMM problem
+        per block pair (``_FusedChain``, the form ``BatchedHeff2`` merges across bonds)."""
         if precision not in ("fp64", "fp32", "bf16x3"):
             raise ValueError(f"unknown precision {precision!r}")
+        if layout not in ("sector", "blocks"):
+            raise ValueError(f"unknown layout {layout!r}")
+        self.layout = layout
         require_device(fl, w_left, w_right, fr)
         self.precision = precision
         self.fl, self.fr = fl, fr
@@ -334,7 +595,11 @@ class Heff2:
     def chain(self, s: TensorStructure):
         c = self._chains.get(s)
         if c is None:
-            if self.fused:
+            if self.fused and self.layout == "sector":
+                wl, wr = self._w
+                key = ("sector", s, self._fr.structure, self._fl.structure, wl[2], wr[2])
+                c = _cached_chain(key, lambda: _SectorChain(s, self._fr.structure, self._fl.structure, wl, wr))
+            elif self.fused:
                 wl, wr = self._w
                 key = ("fused", s, self._fr.structure, self._fl.structure, wl[2], wr[2])
                 c = _cached_chain(key, lambda: _FusedChain(s, self._fr.structure, self._fl.structure,
@@ -351,6 +616,12 @@ class Heff2:
         """``H_eff x`` on a flat buffer of ``structure``; returns a buffer of x's dtype.
         ``events`` (a dict) collects per-step GEMM events for kernel timing."""
         c = self.chain(structure)
+        if self.fused and self.layout == "sector":
+            # FR and FL stacked per sector once; casts ride on the entry and exit passes
+            datas = self._prepared.get(structure)
+            if datas is None:
+                datas = self._prepared[structure] = c.prepare(self._datas)
+            return c.run(x, datas, self.precision, events, dtype=self.dtype)
         if self.fused:
             datas, kw = self._datas, {}
         else:
