@@ -118,13 +118,50 @@ unsigned grid_for(int64_t n) {
 }
 
 // ---- fused two-site MPO application --------------------------------------------------
-// One workgroup owns 1024 positions (r, s) of one slab-pair job.  Every index into the
-// per-thread input array is a compile-time constant (inputs beyond nin are zero and the
-// coefficient rows are zero-padded to kMpoMax columns), so the array lives in VGPRs; the
-// coefficients and slab tables are uniform across the wave and come in through scalar
-// loads.
+// One workgroup owns 1024 positions (r, s) of one slab-pair job.  The job's input count
+// nin picks, once per workgroup, a body compiled for exactly that count: every index into
+// the per-thread input array is a compile-time constant (the array lives in VGPRs) and a
+// position costs nin x nout multiply-adds -- the coefficient rows are zero-padded to
+// kMpoMax columns, but only their first nin entries are read.  Coefficients and slab
+// tables are uniform across the wave and come in through scalar loads.
 constexpr int kMpoMax = 32;   // max input slabs of one job (coefficient row length)
 constexpr int kMpoPer = 4;    // positions per thread
+
+template <typename T, int NIN>
+__device__ __forceinline__ void mpo_body(const T* __restrict__ in, T* __restrict__ out,
+                                         const int64_t* __restrict__ is, const int64_t* __restrict__ os,
+                                         const T* __restrict__ cf, int nout, int64_t base, int64_t n,
+                                         uint32_t S32) {
+  for (int j = 0; j < kMpoPer; ++j) {
+    const int64_t e = base + j * kThreads + threadIdx.x;
+    if (e >= n) break;
+    const uint32_t r = static_cast<uint32_t>(e) / S32;
+    const int64_t s = static_cast<int64_t>(static_cast<uint32_t>(e) - r * S32);
+    T v[NIN > 0 ? NIN : 1];
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) v[k] = in[is[2 * k] + r * is[2 * k + 1] + s];
+    for (int o = 0; o < nout; ++o) {
+      const T* row = cf + o * kMpoMax;
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < NIN; ++k) acc += row[k] * v[k];
+      out[os[2 * o] + r * os[2 * o + 1] + s] = acc;
+    }
+  }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void mpo_dispatch(int nin, const T* __restrict__ in, T* __restrict__ out,
+                                             const int64_t* __restrict__ is, const int64_t* __restrict__ os,
+                                             const T* __restrict__ cf, int nout, int64_t base, int64_t n,
+                                             uint32_t S32) {
+  if constexpr (N <= kMpoMax) {
+    if (nin == N)
+      mpo_body<T, N>(in, out, is, os, cf, nout, base, n, S32);
+    else
+      mpo_dispatch<T, N + 1>(nin, in, out, is, os, cf, nout, base, n, S32);
+  }
+}
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void mpo_kernel(const T* __restrict__ in, T* __restrict__ out,
@@ -139,28 +176,9 @@ __global__ __launch_bounds__(kThreads) void mpo_kernel(const T* __restrict__ in,
   const int64_t R = jw[0], S = jw[1];
   const int nin = static_cast<int>(jw[3] - jw[2]);
   const int nout = static_cast<int>(jw[5] - jw[4]);
-  if (nin > kMpoMax) return;  // the host refuses such tables; never index past v[]
-  const int64_t* is = in_slabs + 2 * jw[2];
-  const int64_t* os = out_slabs + 2 * jw[4];
-  const T* cf = coef + jw[6];
-  const uint32_t S32 = static_cast<uint32_t>(S);
-  const int64_t n = R * S;
-  for (int j = 0; j < kMpoPer; ++j) {
-    const int64_t e = base + j * kThreads + threadIdx.x;
-    if (e >= n) break;
-    const uint32_t r = static_cast<uint32_t>(e) / S32;
-    const int64_t s = static_cast<int64_t>(static_cast<uint32_t>(e) - r * S32);
-    T v[kMpoMax];
-#pragma unroll
-    for (int k = 0; k < kMpoMax; ++k) v[k] = k < nin ? in[is[2 * k] + r * is[2 * k + 1] + s] : T(0);
-    for (int o = 0; o < nout; ++o) {
-      const T* row = cf + o * kMpoMax;
-      T acc = T(0);
-#pragma unroll
-      for (int k = 0; k < kMpoMax; ++k) acc += row[k] * v[k];
-      out[os[2 * o] + r * os[2 * o + 1] + s] = acc;
-    }
-  }
+  if (nin < 0 || nin > kMpoMax) return;  // the host refuses such tables; never index past v[]
+  mpo_dispatch<T, 0>(nin, in, out, in_slabs + 2 * jw[2], out_slabs + 2 * jw[4], coef + jw[6], nout,
+                     base, R * S, static_cast<uint32_t>(S));
 }
 
 }  // namespace

@@ -482,6 +482,7 @@ class _SectorChain:
                                 dense = (o1[0][x] + ix, o1[1][P] + iP, o2[1][Q] + iQ)
                                 outputs.setdefault((a, sg), []).append((off, mx * N, dense, da, ds))
         jobs, ins, outs, coef, tiles = _mpo_jobs(inputs, outputs, w12, max(t1_acc, 1), max(t3_acc, 1))
+        self.mpo_host = (jobs, ins, outs, coef)
         self.mpo_njobs = len(jobs)
         tiles = np.concatenate(tiles) if tiles else np.zeros((0, 2), np.int64)
         self.mpo_ntiles = len(tiles)
