@@ -316,6 +316,47 @@ def choose_tile(Ms, Ns, Ks) -> tuple:
     return (64, 64)
 
 
+def _xcd_layout(tiles: np.ndarray, nprob: int, nxcd: int = 8) -> np.ndarray:
+    """Tile table for a launch of one workgroup per tile on a device whose workgroup b
+    runs on XCD ``b % nxcd`` and whose GEMM kernel hands XCD x the table slice
+    ``[x n / nxcd, (x + 1) n / nxcd)``.
+
+    Tiles (rows ``(problem, tile row, tile col, K)``) are cut, in row-major order within each
+    problem, into units of about 1/48 of the total work -- neighbouring tiles share operand
+    panels, so each unit stays on one XCD's L2; units go to the least loaded XCD, largest first
+    (LPT); every XCD runs its tiles longest K first.  The slices are padded to equal
+    length with tiles of problem ``nprob`` (an empty problem row)."""
+    work = tiles[:, 3].astype(np.float64) + 64.0  # K plus a per-tile fixed cost
+    cap = max(work.sum() / (nxcd * 6), 1.0)
+    order = np.lexsort((tiles[:, 2], tiles[:, 1], tiles[:, 0]))
+    units, cur, cw, key = [], [], 0.0, None
+    for idx in order:
+        p, ti = int(tiles[idx, 0]), int(tiles[idx, 1])
+        if cur and (p != key[0] or cw >= cap):
+            units.append(cur)
+            cur, cw = [], 0.0
+        cur.append(idx)
+        cw += work[idx]
+        key = (p, ti)
+    if cur:
+        units.append(cur)
+    units.sort(key=lambda u: -work[u].sum())
+    loads = np.zeros(nxcd)
+    parts = [[] for _ in range(nxcd)]
+    for u in units:
+        x = int(np.argmin(loads))
+        parts[x].extend(u)
+        loads[x] += work[u].sum()
+    L = max(len(pt) for pt in parts)
+    out = np.zeros((nxcd * L, 4), dtype=tiles.dtype)
+    out[:, 0] = nprob
+    for x, pt in enumerate(parts):
+        pt = np.asarray(pt, dtype=np.int64)
+        pt = pt[np.argsort(-tiles[pt, 3], kind="stable")]
+        out[x * L: x * L + len(pt)] = tiles[pt]
+    return out
+
+
 class GemmProgram:
     """A compiled ``tn_gemm_grouped`` launch over a list of block problems.
 
@@ -334,6 +375,7 @@ class GemmProgram:
 
     SPLIT_TARGET_TILES = 1536
     SPLIT_MIN_K = 256
+    XCD_LAYOUT = "lpt"  # tile table layout over the 8 XCDs: "lpt", "deal" or "none"
     SKINNY_MAX_W = 8    # the kernel's register line width (kSkinnyMax)
     SKINNY_MAX_K = 64
 
@@ -453,16 +495,19 @@ class GemmProgram:
                 rows.append(np.stack([np.full(tm * tn, p), ii.ravel(), jj.ravel(),
                                       np.full(tm * tn, self.Ks[p])], 1))
             tiles = np.concatenate(rows) if rows else np.zeros((0, 4), np.int64)
-            # longest K first; ties keep problem order
-            tiles = tiles[np.argsort(-tiles[:, 3], kind="stable")]
-            if len(tiles) % 8 == 0:
-                # the kernel hands XCD x the table slice [x n/8, (x+1) n/8) (workgroup b
-                # runs on XCD b % 8); deal the sorted tiles round-robin into those slices
-                # so every XCD gets the same mix of long and short tiles, longest first
-                tiles = tiles[np.arange(len(tiles)).reshape(-1, 8).T.reshape(-1)]
+            if self.XCD_LAYOUT == "lpt" and len(tiles) >= 64:
+                tiles = _xcd_layout(tiles, self.nproblems)
+            else:
+                # longest K first; ties keep problem order
+                tiles = tiles[np.argsort(-tiles[:, 3], kind="stable")]
+                if self.XCD_LAYOUT == "deal" and len(tiles) % 8 == 0:
+                    tiles = tiles[np.arange(len(tiles)).reshape(-1, 8).T.reshape(-1)]
             tiles[:, 3] = 0
             tiles = tiles.astype(np.int32)
-            d = _DeviceCache(probs=self._probs.reshape(-1), segs=self._segs.reshape(-1),
+            # one all-zero problem row past the real ones: the padding tiles of the XCD
+            # layout point at it (M = N = 0, no segments: the workgroup exits at once)
+            probs = np.vstack([self._probs, np.zeros((1, PROB_WORDS), np.int64)])
+            d = _DeviceCache(probs=probs.reshape(-1), segs=self._segs.reshape(-1),
                              tiles=tiles.reshape(-1))
             d.ntiles = len(tiles)
             self._by_tile[tile] = d
