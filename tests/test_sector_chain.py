@@ -1,0 +1,143 @@
+"""Host logic of the sector-stacked two-site H_eff (``tenet.network.heff._SectorChain``):
+its compiled tables -- entry gather, FR/FL stacking, the two GEMM problem lists, the MPO
+slab jobs and the exit scatter -- are executed here by a NumPy interpreter of the kernels'
+documented semantics and compared with the FP64 oracle of the reference's ``_heff2_full``
+(``network/env.py:416-432``).  No device is needed: any wrong offset, stride, coefficient or
+missing term in the tables fails these tests on the CPU."""
+
+import numpy as np
+import pytest
+
+from oracle import u1_network as N
+from tests.helpers import rel_err, to_oracle
+
+
+def run_recouple(prog, src, dst):
+    """``dst[region] = sum_t coeff_t * src[term_t + permuted index]`` for every job."""
+    from tenet.engine import JOB_WORDS
+
+    h = prog._tables.host
+    jobs = np.asarray(h["jobs"]).reshape(-1, JOB_WORDS)
+    for j in jobs:
+        doff, tb, te, rank = (int(v) for v in j[:4])
+        shape, dstr, sstr = j[4:4 + rank], j[10:10 + rank], j[16:16 + rank]
+        idx = np.indices(shape).reshape(rank, -1)
+        d = doff + (dstr[:, None] * idx).sum(0)
+        s = (sstr[:, None] * idx).sum(0)
+        acc = np.zeros(d.shape[0])
+        for t in range(tb, te):
+            acc += h["tco"][t] * src[h["tsrc"][t] + s]
+        dst[d] = acc
+
+
+def run_gemm(g, A, B, C):
+    """``C_p = sum_s op(A_s) op(B_s)`` over the compiled problem and segment tables; flagged
+    problems write the workspace (allocated here), then the program's own reduction runs."""
+    W = np.zeros(max(g.ws_size, 1))
+    for M, N_, c_off, ldc, sb, se, flag, _ in g._probs:
+        acc = np.zeros((M, N_))
+        for a_off, b_off, K, lda, ldb, _ in g._segs[sb:se]:
+            m, n, k = np.arange(M)[:, None], np.arange(N_)[None, :], np.arange(K)
+            Am = A[a_off + k[None, :] * lda + m] if g.trans_a else A[a_off + m * lda + k[None, :]]
+            Bm = B[b_off + n * ldb + k[:, None]] if g.trans_b else B[b_off + k[:, None] * ldb + n]
+            acc += Am @ Bm
+        out = W if flag else C
+        out[c_off + np.arange(M)[:, None] * ldc + np.arange(N_)[None, :]] = acc
+    if g.reduce is not None:
+        run_recouple(g.reduce, W, C)
+    return W
+
+
+def run_mpo(jobs, ins, outs, coef, t1, t3, width=32):
+    for R, S, ib, ie, ob, oe, cf, _ in jobs:
+        r, s = np.arange(R)[:, None], np.arange(S)[None, :]
+        v = [t1[off + r * rs + s] for off, rs in ins[ib:ie]]
+        for o, (off, rs) in enumerate(outs[ob:oe]):
+            acc = np.zeros((R, S))
+            for k, vk in enumerate(v):
+                acc += coef[cf + o * width + k] * vk
+            t3[off + r * rs + s] = acc
+
+
+def emulate(ch, fl, fr, aa):
+    x = aa.data.double().numpy()
+    xs = np.zeros(max(ch.xs_size, 1))
+    run_recouple(ch.entry, x, xs)
+    frs = np.zeros(max(ch.frs_size, 1))
+    run_recouple(ch.fr_stack, fr.data.double().numpy(), frs)
+    fls = np.zeros(max(ch.fls_size, 1))
+    run_recouple(ch.fl_stack, fl.data.double().numpy(), fls)
+    t1 = np.zeros(max(ch.t1_size, 1))
+    run_gemm(ch.gemm1, xs, frs, t1)
+    t3 = np.zeros(max(ch.t3_size, 1))
+    run_mpo(*ch.mpo_host, t1, t3)
+    ws = run_gemm(ch.gemm4, fls, t3, None)
+    out = np.zeros(aa.structure.size)
+    run_recouple(ch.exit, ws, out)
+    return out
+
+
+def _problem(left, right, n_sites, dim, sigma, seed):
+    from tenet.network.models import MPO_BOND, PHYS, gaussian_bond, heisenberg_mpo
+    from tenet.space import IN, OUT, Leg
+    from tenet.tensor import SymmetricTensor
+
+    bl = gaussian_bond(left, n_sites, dim, sigma)
+    br = gaussian_bond(right, n_sites, dim, sigma)
+    h = heisenberg_mpo(n_sites, 0.8, device="cpu")
+    fl = SymmetricTensor.random((Leg(bl, IN), Leg(MPO_BOND, OUT), Leg(bl, OUT)), seed=seed, device="cpu")
+    fr = SymmetricTensor.random((Leg(br, OUT), Leg(MPO_BOND, IN), Leg(br, IN)), seed=seed + 1, device="cpu")
+    aa = SymmetricTensor.random((Leg(bl, OUT), Leg(PHYS, OUT), Leg(PHYS, OUT), Leg(br, IN)),
+                                seed=seed + 2, device="cpu")
+    return fl, h[left], h[left + 1], fr, aa
+
+
+def _chain(fl, w1, w2, fr, aa):
+    from tenet.network.heff import _SectorChain, _w_host
+
+    return _SectorChain(aa.structure, fr.structure, fl.structure, _w_host(w1), _w_host(w2))
+
+
+@pytest.mark.parametrize("left,right,n_sites,dim,sigma,seed", [
+    (5, 7, 12, 40, 1.5, 3),      # centre bond, symmetric sector sets
+    (2, 4, 12, 64, 3.0, 11),     # left bond 3 sectors, right bond 5
+    (9, 11, 20, 512, 4.0, 5),    # step 3 split into K slices
+])
+def test_sector_chain_tables_reproduce_the_oracle(left, right, n_sites, dim, sigma, seed):
+    fl, w1, w2, fr, aa = _problem(left, right, n_sites, dim, sigma, seed)
+    ch = _chain(fl, w1, w2, fr, aa)
+    got = emulate(ch, fl, fr, aa)
+    ref = N.heff2(*(to_oracle(t) for t in (fl, w1, w2, fr, aa)))
+    assert rel_err(got, ref.flat()) < 1e-12
+
+
+def test_sector_chain_counts_the_same_work_as_the_block_chain():
+    from tenet.network.heff import _FusedChain, _w_host
+
+    fl, w1, w2, fr, aa = _problem(9, 11, 20, 512, 4.0, 5)
+    sec = _chain(fl, w1, w2, fr, aa)
+    blk = _FusedChain(aa.structure, fr.structure, fl.structure, _w_host(w1), _w_host(w2), "rys")
+    assert sec.flops == blk.flops
+    assert sec.gemm1.flops == blk.ex1.flops and sec.gemm4.flops == blk.gemm4.flops
+    # one GEMM problem per charge sector instead of one per block pair
+    assert len(sec.gemm1.problems) < len(blk.ex1.gemm.problems)
+    assert len(sec.gemm4.problems) < len(blk.gemm4.problems)
+    # step 3 is split into K slices and the exit pass reads every slice once
+    assert sec.gemm4.ws_size > aa.structure.size
+    assert sec.exit.moved_terms == sec.gemm4.ws_size
+    assert sec.exit.elements == aa.structure.size
+
+
+def test_planted_mpo_coefficient_error_is_caught():
+    """The interpreter is not vacuous: one wrong MPO coefficient moves the result."""
+    fl, w1, w2, fr, aa = _problem(5, 7, 12, 40, 1.5, 3)
+    ch = _chain(fl, w1, w2, fr, aa)
+    jobs, ins, outs, coef = ch.mpo_host
+    coef = list(coef)
+    k = next(i for i, c in enumerate(coef) if c != 0.0)
+    coef[k] *= -1.0
+    ch.mpo_host = (jobs, ins, outs, coef)
+    got = emulate(ch, fl, fr, aa)
+    ref = N.heff2(*(to_oracle(t) for t in (fl, w1, w2, fr, aa)))
+    # far outside the 1e-12 agreement of the correct tables (the flipped slab is small)
+    assert rel_err(got, ref.flat()) > 1e-9
