@@ -40,7 +40,7 @@ PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16x3": 2500.0}
 # scaled by what the same counters read for `bench.py --workload copy` (a 1 GiB device
 # copy: FETCH_SIZE 524302 KiB for 1048576 KiB read, WRITE_SIZE 1048576 KiB written).
 PMC_TRAFFIC = {
-    (64, 2048, "fp32"): {"fetch_kib": {0: 56560.1, 3: 99727.2}, "write_kib": {0: 33104.9, 3: 19810.1},
+    (64, 2048, "fp32"): {"fetch_kib": {0: 28252.6, 3: 37311.1}, "write_kib": {0: 30295.2, 3: 23977.9},
                          "fetch_scale": 0.5, "write_scale": 1.0,
                          "source": "profiles/heff2_L64_D2048_fp32_traffic.md"},
 }
