@@ -91,11 +91,13 @@ __global__ __launch_bounds__(kThreads) void recouple_kernel(const TS* __restrict
     return;
   }
 
-  // LDS transpose tile over (I = rank-1, J = mode); remaining dims are batch.
+  // LDS transpose tile over (I = rank-1, J = mode); remaining dims are batch.  Mode J + 8
+  // asks for a 32x32 tile (four times the workgroups for a program with few large regions).
+  const int E = mode >= 8 ? kT / 2 : kT;
   const int I = rank - 1;
-  const int J = static_cast<int>(mode);
-  const int64_t nI = (shape[I] + kT - 1) / kT;
-  const int64_t nJ = (shape[J] + kT - 1) / kT;
+  const int J = static_cast<int>(mode >= 8 ? mode - 8 : mode);
+  const int64_t nI = (shape[I] + E - 1) / E;
+  const int64_t nJ = (shape[J] + E - 1) / E;
   const int64_t ti = tile % nI;
   const int64_t tj = (tile / nI) % nJ;
   uint32_t b = static_cast<uint32_t>(tile / (nI * nJ));
@@ -109,22 +111,22 @@ __global__ __launch_bounds__(kThreads) void recouple_kernel(const TS* __restrict
     sbase += i * sstr[d];
     dbase += i * dstr[d];
   }
-  const int64_t i0 = ti * kT, j0 = tj * kT;
-  const int64_t ni = min<int64_t>(kT, shape[I] - i0);
-  const int64_t nj = min<int64_t>(kT, shape[J] - j0);
+  const int64_t i0 = ti * E, j0 = tj * E;
+  const int64_t ni = min<int64_t>(E, shape[I] - i0);
+  const int64_t nj = min<int64_t>(E, shape[J] - j0);
   const int64_t sI = sstr[I], sJ = sstr[J], dJ = dstr[J], dI = dstr[I];
   // load: threads run along J (source-contiguous)
-  for (int r = 0; r < kT * kT / kThreads; ++r) {
+  for (int r = 0; r < E * E / kThreads; ++r) {
     const int idx = r * kThreads + tid;
-    const int j = idx % kT, i = idx / kT;
+    const int j = idx % E, i = idx / E;
     if (i < ni && j < nj)
       lds[i][j] = gather_terms<T>(src, tsrc, tco, tb, te, sbase + (i0 + i) * sI + (j0 + j) * sJ);
   }
   __syncthreads();
   // store: threads run along I (destination-contiguous)
-  for (int r = 0; r < kT * kT / kThreads; ++r) {
+  for (int r = 0; r < E * E / kThreads; ++r) {
     const int idx = r * kThreads + tid;
-    const int i = idx % kT, j = idx / kT;
+    const int i = idx % E, j = idx / E;
     if (i < ni && j < nj) dst[dst_off + dbase + (j0 + j) * dJ + (i0 + i) * dI] = static_cast<TD>(lds[i][j]);
   }
 }

@@ -47,6 +47,7 @@ __all__ = [
 
 _LIN_TILE = 2048
 _LDS_T = 64
+_SMALL_PROGRAM_TILES = 2048  # below this many workgroups, LDS jobs use 32 x 32 tiles
 _GEMM_PRECISION = {"value": os.environ.get("TENET_GEMM_PRECISION", "fp32")}
 
 
@@ -157,7 +158,7 @@ class RecoupleProgram:
             else:
                 batch = nelem // (shp[-1] * shp[mode])
                 nt = (-(-shp[-1] // _LDS_T)) * (-(-shp[mode] // _LDS_T)) * batch
-            tiles.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
+            tiles.append((j, nt, mode, shp))
             self.elements += nelem
             self.moved_terms += nelem * len(terms)
             # every access stays in bounds: checked here once, on the host
@@ -169,7 +170,17 @@ class RecoupleProgram:
                 if so < 0 or so + span >= src_size:
                     raise ValueError("recouple term reads outside its source buffer")
         self.njobs = n
-        tiles_arr = np.concatenate(tiles) if tiles else np.zeros((0, 2), np.int64)
+        # a program of few large transpose regions would leave most of the device idle:
+        # give its LDS jobs 32 x 32 tiles (mode word J + 8), four times the workgroups
+        small = sum(nt for _, nt, _, _ in tiles) < _SMALL_PROGRAM_TILES
+        tl = []
+        for j, nt, mode, shp in tiles:
+            if mode >= 0 and small:
+                jobs[j, 22] = mode + 8
+                e = _LDS_T // 2
+                nt = (-(-shp[-1] // e)) * (-(-shp[mode] // e)) * (int(np.prod(shp)) // (shp[-1] * shp[mode]))
+            tl.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
+        tiles_arr = np.concatenate(tl) if tl else np.zeros((0, 2), np.int64)
         self.ntiles = len(tiles_arr)
         self.src_size, self.dst_size = src_size, dst_size
         self._tables = _DeviceCache(
@@ -620,6 +631,7 @@ class ContractionExec:
         cp = contraction_plan(sa, sb, axes)
         rp_a = repartition_plan(sa, cp.a_outputs, cp.a_inputs)
         rp_b = repartition_plan(sb, cp.b_outputs, cp.b_inputs)
+        self.a_plan = (tuple(rp_a.perm), tuple(rp_a.terms))  # sa -> the GEMM's A layout
         self.a = _operand(sa, rp_a)
         self.b = _operand(sb, rp_b)
         Ap, Bp = self.a.structure, self.b.structure
@@ -639,6 +651,7 @@ class ContractionExec:
             cur = t.new_structure
         self.out_structure = cur
         self.c_structure = Cp
+        self.c_plan = (tuple(plan[0]), tuple(plan[1]))  # the GEMM's C layout -> out
         c_offs = scatter_map(Cp, cur, plan[0], plan[1])
         if c_offs is None:
             self.c_program = recouple_program(Cp, cur, tuple(plan[0]), tuple(plan[1]))

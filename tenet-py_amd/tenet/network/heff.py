@@ -51,56 +51,124 @@ def _einsum_axes(t0: str, t1: str, out: str):
     return axes, tuple(free.index(c) for c in out)
 
 
+def _identity_plan(S: TensorStructure):
+    return tuple(range(S.ndim)), tuple((i, i, 1.0) for i in range(S.num_blocks))
+
+
+def _boundary(S: TensorStructure, D: TensorStructure, plan):
+    """One recouple program for a composed (perm, terms) map from S's layout to D's, or
+    None when the map leaves every block where it is (the next launch reads in place)."""
+    perm, terms = plan
+    offs = inplace_map(S, D, perm, terms)
+    if offs is not None and (offs == D.layout.offsets).all():
+        return None
+    return recouple_program(S, D, tuple(perm), tuple(terms))
+
+
 class _Chain:
     """The compiled launch sequence for one input structure.  ``statics`` holds the
-    ``(structure, term)`` of each step's static operand; their buffers come at run time."""
+    ``(structure, term)`` of each step's static operand; their buffers come at run time.
+
+    Between two GEMMs the data passes through at most ONE recouple launch: the previous
+    step's output scatter (C layout -> output blocks), the next step's bend (the
+    ``_composed`` repartition, ``env.py:60-90``) and its operand gather (-> A layout) are
+    composed on the host into one (perm, terms) map.  A step whose K is split writes its
+    slices to a workspace and the slice sum joins the final scatter; the FP32/BF16x3 casts of
+    the Lanczos vector ride on the first and last launch when there is one."""
 
     def __init__(self, s_in: TensorStructure, statics) -> None:
-        self.steps = []
+        from tenet.engine import GemmProgram, RecoupleProgram, compose_plans, regions_for_plan
+
+        self.stages = []  # (boundary program or None, A-layout structure, ContractionExec)
         self.flops = 0
+        src, src_plan = s_in, _identity_plan(s_in)  # data layout, and its map to tensor s
         s = s_in
         for (eq, bend), stat in zip(HEFF2_STEPS, statics):
             lhs, out = eq.split("->")
             ta, tb = lhs.split(",")
-            rep = None
+            plan = src_plan
             if bend:
                 outs, ins, ta = _bent(s, ta, bend)
-                plan = repartition_plan(s, outs, ins)
-                offs = inplace_map(s, plan.new_structure, plan.perm, plan.terms)
-                if offs is None or not (offs == plan.new_structure.layout.offsets).all():
-                    rep = recouple_program(s, plan.new_structure, tuple(plan.perm), tuple(plan.terms))
-                s = plan.new_structure
+                bp = repartition_plan(s, outs, ins)
+                plan = compose_plans(plan, (bp.perm, bp.terms))
+                s = bp.new_structure
             axes, out_axes = _einsum_axes(ta, stat[1], out)
             ex = contraction_exec(s, stat[0], axes, out_axes)
+            if ex.a.program is not None:
+                plan = compose_plans(plan, ex.a_plan)
+                a_struct = ex.a.structure
+            else:
+                a_struct = s  # the GEMM reads A in place from s's layout
+            self.stages.append((_boundary(src, a_struct, plan), a_struct, ex))
             self.flops += ex.flops
-            self.steps.append((rep, s, ex))
+            if ex.c_program is not None:
+                src, src_plan = ex.c_structure, ex.c_plan
+            else:
+                src, src_plan = ex.out_structure, _identity_plan(ex.out_structure)
             s = ex.out_structure
         if s != s_in:
             raise ValueError("heff2: the output structure differs from the input's")
         self.s_out = s
+        # the last GEMM: when its K is split, gather the slices and sum them in the final pass
+        ex = self.stages[-1][2]
+        self.last_gemm = ex.gemm
+        if ex.gemm.reduce is not None:
+            g = ex.gemm
+            self.last_gemm = GemmProgram(g.problems, g.a_size, g.b_size, 1, g.trans_a, g.trans_b,
+                                         tile=g.tile, gather=True)
+            by_off = {int(p[2]): k for k, p in enumerate(g.problems)}
+            regions = []
+            for doff, shape, dstr, sstr, terms in regions_for_plan(src, s, *src_plan):
+                sl = [(o, c) for so, c in terms for o in self.last_gemm.slices[by_off[so]]]
+                regions.append((doff, shape, dstr, sstr, sl))
+            self.final = RecoupleProgram(regions, max(self.last_gemm.ws_size, 1), s.size)
+        else:
+            self.final = _boundary(src, s, src_plan)
 
     def prepare(self, datas) -> tuple:
         """The static operands in the layouts the step GEMMs read (done once per operator)."""
-        return tuple(ex.prepare_b(d) for (_, _, ex), d in zip(self.steps, datas))
+        return tuple(ex.prepare_b(d) for (_, _, ex), d in zip(self.stages, datas))
 
     def run(self, x: torch.Tensor, datas, precision: str, events: dict | None = None,
-            prepared: bool = False) -> torch.Tensor:
+            prepared: bool = False, dtype: torch.dtype | None = None) -> torch.Tensor:
+        """``H_eff x`` with the GEMMs in ``dtype`` (default: x's); returns x's type."""
+        wt = dtype or x.dtype
         cur = x
-        for k, ((rep, s, ex), data) in enumerate(zip(self.steps, datas)):
-            if rep is not None:
-                nxt = torch.empty(s.size, dtype=cur.dtype, device=cur.device)
-                if events is not None:
-                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                    e0.record()
-                rep.run(cur, nxt)
-                if events is not None:
-                    e1.record()
-                    events.setdefault(f"recouple{k}", []).append((e0, e1, rep.bytes_moved(cur.element_size())))
+        n = len(self.stages)
+
+        def timed(name, prog, src, dst):
+            if events is None:
+                prog.run(src, dst)
+                return
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            prog.run(src, dst)
+            e1.record()
+            events.setdefault(name, []).append((e0, e1, prog.bytes_moved(dst.element_size())))
+
+        for k, ((prog, a_struct, ex), data) in enumerate(zip(self.stages, datas)):
+            if prog is not None:
+                nxt = torch.empty(a_struct.size, dtype=wt, device=x.device)
+                timed(f"recouple{k}", prog, cur, nxt)
                 cur = nxt
+            elif cur.dtype != wt:
+                cur = cur.to(wt)
+            bp = data if prepared else ex.prepare_b(data)
             ev = None if events is None else events.setdefault(k, [])
             rev = None if events is None else events.setdefault(f"reduce{k}", [])
-            cur = ex.run(cur, data, precision, events=ev, reduce_events=rev, b_ready=prepared)
-        return cur
+            if k == n - 1 and self.last_gemm is not ex.gemm:
+                target = torch.empty(max(self.last_gemm.ws_size, 1), dtype=wt, device=x.device)
+                self.last_gemm.run(cur, bp, target, precision, events=ev)
+            else:
+                # the C layout (or, when the GEMM scatters in place, the output's)
+                target = torch.empty(max(ex.gemm.c_size, 1), dtype=wt, device=x.device)
+                ex.gemm.run(cur, bp, target, precision, events=ev, reduce_events=rev)
+            cur = target
+        if self.final is not None:
+            out = torch.empty(self.s_out.size, dtype=x.dtype, device=x.device)
+            timed("final", self.final, cur, out)
+            return out
+        return cur if cur.dtype == x.dtype else cur.to(x.dtype)
 
 
 def _dense(t: SymmetricTensor):
@@ -623,14 +691,14 @@ class Heff2:
             if datas is None:
                 datas = self._prepared[structure] = c.prepare(self._datas)
             return c.run(x, datas, self.precision, events, dtype=self.dtype)
-        if self.fused:
-            datas, kw = self._datas, {}
-        else:
-            # static operands recoupled once into the layouts the step GEMMs read
+        if not self.fused:
+            # static operands recoupled once into the layouts the step GEMMs read; casts ride
+            # on the chain's first and last recouple launch
             datas = self._prepared.get(structure)
             if datas is None:
                 datas = self._prepared[structure] = c.prepare(self._datas)
-            kw = {"prepared": True}
+            return c.run(x, datas, self.precision, events, prepared=True, dtype=self.dtype)
+        datas, kw = self._datas, {}
         if x.dtype != self.dtype:
             y = c.run(ops.astype(SymmetricTensor(structure, x), self.dtype).data, datas,
                       self.precision, events, **kw)
