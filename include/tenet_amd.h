@@ -49,8 +49,9 @@ extern "C" {
 /* Recouple job table: TN_JOB_WORDS int64 per destination region.
  *   [0] dst_off   [1] term_begin  [2] term_end  [3] rank (<= TN_MAX_RANK)
  *   [4..9] shape  [10..15] dst_stride  [16..21] src_stride (dims in dst C order)
- *   [22] mode: -1 = linear (dst-ordered), else the index J of the src-contiguous dim used
- *        as the second axis of a 64x64 LDS transpose tile (J + 8: a 32x32 tile)
+ *   [22] mode: -1 = linear (dst-ordered, 2048 elements per tile; -2: 512 per tile), else
+ *        the index J of the src-contiguous dim used as the second axis of a 64x64 LDS
+ *        transpose tile (J + 8: a 32x32 tile)
  *   [23] element count
  * Term table: per term one int64 source offset (elements) and one double coefficient.
  * Tile table: 2 int64 per tile: (job, tile index within the job). */

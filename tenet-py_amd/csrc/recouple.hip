@@ -29,6 +29,51 @@ __device__ __forceinline__ T gather_terms(const TS* __restrict__ src, const int6
   return acc;
 }
 
+// Linear tile: element e of the job in destination C order, PER elements per thread.  All
+// of a thread's elements are decoded first and their loads issued together (independent,
+// one round of memory latency per term instead of one per element).
+template <typename T, typename TS, typename TD, int PER>
+__device__ __forceinline__ void linear_tile(const TS* __restrict__ src, TD* __restrict__ dst,
+                                            const int64_t* __restrict__ tsrc,
+                                            const double* __restrict__ tco, int64_t tb, int64_t te,
+                                            int rank, const int64_t* shape, const int64_t* dstr,
+                                            const int64_t* sstr, int64_t dst_off, int64_t nelem,
+                                            int64_t tile) {
+  const int tid = threadIdx.x;
+  const int64_t base = tile * (kThreads * PER);
+  int64_t so[PER], dof[PER];
+  bool live[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int64_t e = base + r * kThreads + tid;
+    live[r] = e < nelem;
+    so[r] = 0;
+    dof[r] = 0;
+    uint32_t rem = live[r] ? static_cast<uint32_t>(e) : 0u;
+    for (int d = rank - 1; d >= 0; --d) {
+      const uint32_t n = static_cast<uint32_t>(shape[d]);
+      const uint32_t q = rem / n;
+      const int64_t i = rem - q * n;
+      rem = q;
+      so[r] += i * sstr[d];
+      dof[r] += i * dstr[d];
+    }
+  }
+  T acc[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) acc[r] = T(0);
+  for (int64_t t = tb; t < te; ++t) {
+    const T c = static_cast<T>(tco[t]);
+    const TS* __restrict__ s0 = src + tsrc[t];
+#pragma unroll
+    for (int r = 0; r < PER; ++r)
+      if (live[r]) acc[r] += c * static_cast<T>(s0[so[r]]);
+  }
+#pragma unroll
+  for (int r = 0; r < PER; ++r)
+    if (live[r]) dst[dst_off + dof[r]] = static_cast<TD>(acc[r]);
+}
+
 // TS: source element type, TD: destination element type (a cast rides along with the
 // permutation); T: the arithmetic type, the wider of the two.
 template <typename TS, typename TD>
@@ -53,41 +98,12 @@ __global__ __launch_bounds__(kThreads) void recouple_kernel(const TS* __restrict
   const int tid = threadIdx.x;
 
   if (mode < 0) {
-    // linear: element e of the job in destination C order.  All of a thread's elements
-    // are decoded first and their loads issued together (independent, one round of
-    // memory latency per term instead of one per element).
-    const int64_t base = tile * kLinTile;
-    int64_t so[kLinPerThread], dof[kLinPerThread];
-    bool live[kLinPerThread];
-#pragma unroll
-    for (int r = 0; r < kLinPerThread; ++r) {
-      const int64_t e = base + r * kThreads + tid;
-      live[r] = e < nelem;
-      so[r] = 0;
-      dof[r] = 0;
-      uint32_t rem = live[r] ? static_cast<uint32_t>(e) : 0u;
-      for (int d = rank - 1; d >= 0; --d) {
-        const uint32_t n = static_cast<uint32_t>(shape[d]);
-        const uint32_t q = rem / n;
-        const int64_t i = rem - q * n;
-        rem = q;
-        so[r] += i * sstr[d];
-        dof[r] += i * dstr[d];
-      }
-    }
-    T acc[kLinPerThread];
-#pragma unroll
-    for (int r = 0; r < kLinPerThread; ++r) acc[r] = T(0);
-    for (int64_t t = tb; t < te; ++t) {
-      const T c = static_cast<T>(tco[t]);
-      const TS* __restrict__ s0 = src + tsrc[t];
-#pragma unroll
-      for (int r = 0; r < kLinPerThread; ++r)
-        if (live[r]) acc[r] += c * static_cast<T>(s0[so[r]]);
-    }
-#pragma unroll
-    for (int r = 0; r < kLinPerThread; ++r)
-      if (live[r]) dst[dst_off + dof[r]] = static_cast<TD>(acc[r]);
+    // linear tiles: 2048 elements (mode -1) or, for programs with few elements, 512 (mode -2)
+    if (mode == -2)
+      linear_tile<T, TS, TD, 2>(src, dst, tsrc, tco, tb, te, rank, shape, dstr, sstr, dst_off, nelem, tile);
+    else
+      linear_tile<T, TS, TD, kLinPerThread>(src, dst, tsrc, tco, tb, te, rank, shape, dstr, sstr, dst_off,
+                                            nelem, tile);
     return;
   }
 

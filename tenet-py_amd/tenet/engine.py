@@ -170,8 +170,9 @@ class RecoupleProgram:
                 if so < 0 or so + span >= src_size:
                     raise ValueError("recouple term reads outside its source buffer")
         self.njobs = n
-        # a program of few large transpose regions would leave most of the device idle:
-        # give its LDS jobs 32 x 32 tiles (mode word J + 8), four times the workgroups
+        # a program of few large regions would leave most of the device idle: give its LDS
+        # jobs 32 x 32 tiles, four times the workgroups
+        # (mode word J + 8); its linear jobs likewise get 512-element tiles (mode word -2)
         small = sum(nt for _, nt, _, _ in tiles) < _SMALL_PROGRAM_TILES
         tl = []
         for j, nt, mode, shp in tiles:
@@ -179,6 +180,9 @@ class RecoupleProgram:
                 jobs[j, 22] = mode + 8
                 e = _LDS_T // 2
                 nt = (-(-shp[-1] // e)) * (-(-shp[mode] // e)) * (int(np.prod(shp)) // (shp[-1] * shp[mode]))
+            elif mode < 0 and small:
+                jobs[j, 22] = -2
+                nt = -(-int(np.prod(shp)) // (_LIN_TILE // 4))
             tl.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
         tiles_arr = np.concatenate(tl) if tl else np.zeros((0, 2), np.int64)
         self.ntiles = len(tiles_arr)
