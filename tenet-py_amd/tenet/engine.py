@@ -331,7 +331,7 @@ def choose_tile(Ms, Ns, Ks) -> tuple:
     return (64, 64)
 
 
-def _xcd_layout(tiles: np.ndarray, nprob: int, nxcd: int = 8) -> np.ndarray:
+def _xcd_layout(tiles: np.ndarray, nprob: int, nxcd: int = 8, grid: int = 0) -> np.ndarray:
     """Tile table for a launch of one workgroup per tile on a device whose workgroup b
     runs on XCD ``b % nxcd`` and whose GEMM kernel hands XCD x the table slice
     ``[x n / nxcd, (x + 1) n / nxcd)``.
@@ -340,7 +340,11 @@ def _xcd_layout(tiles: np.ndarray, nprob: int, nxcd: int = 8) -> np.ndarray:
     problem, into units of about 1/48 of the total work -- neighbouring tiles share operand
     panels, so each unit stays on one XCD's L2; units go to the least loaded XCD, largest first
     (LPT); every XCD runs its tiles longest K first.  The slices are padded to equal
-    length with tiles of problem ``nprob`` (an empty problem row)."""
+    length with tiles of problem ``nprob`` (an empty problem row).
+
+    ``grid`` > 0 (a multiple of ``nxcd``): the layout of a persistent launch of ``grid``
+    workgroups, where workgroup i of XCD x walks table positions ``x grid/nxcd + i + k grid``
+    (k = 0, 1, ...); each XCD's longest-first list is dealt round-robin over its workgroups."""
     work = tiles[:, 3].astype(np.float64) + 64.0  # K plus a per-tile fixed cost
     cap = max(work.sum() / (nxcd * 6), 1.0)
     order = np.lexsort((tiles[:, 2], tiles[:, 1], tiles[:, 0]))
@@ -362,6 +366,17 @@ def _xcd_layout(tiles: np.ndarray, nprob: int, nxcd: int = 8) -> np.ndarray:
         x = int(np.argmin(loads))
         parts[x].extend(u)
         loads[x] += work[u].sum()
+    if grid > 0:
+        per = grid // nxcd
+        rounds = max(-(-len(pt) // per) for pt in parts)
+        out = np.zeros((grid * rounds, 4), dtype=tiles.dtype)
+        out[:, 0] = nprob
+        for x, pt in enumerate(parts):
+            pt = np.asarray(pt, dtype=np.int64)
+            pt = pt[np.argsort(-tiles[pt, 3], kind="stable")]
+            j = np.arange(len(pt))
+            out[x * per + j % per + (j // per) * grid] = tiles[pt]
+        return out
     L = max(len(pt) for pt in parts)
     out = np.zeros((nxcd * L, 4), dtype=tiles.dtype)
     out[:, 0] = nprob
@@ -487,10 +502,11 @@ class GemmProgram:
             chunks.append(cur)
         return chunks
 
-    def _tables(self, tile, device):
+    def _tables(self, tile, device, grid: int = 0):
         if self.skinny is not None:
             tile = "skinny"
-        d = self._by_tile.get(tile)
+        key = tile if grid <= 0 else (tile, grid)
+        d = self._by_tile.get(key)
         if d is None and tile == "skinny":
             lines = self.Ms if self.skinny else self.Ns
             tiles = np.asarray([(p, b) for p, n in enumerate(lines) for b in range(-(-n // 256))],
@@ -510,7 +526,9 @@ class GemmProgram:
                 rows.append(np.stack([np.full(tm * tn, p), ii.ravel(), jj.ravel(),
                                       np.full(tm * tn, self.Ks[p])], 1))
             tiles = np.concatenate(rows) if rows else np.zeros((0, 4), np.int64)
-            if self.XCD_LAYOUT == "lpt" and len(tiles) >= 64:
+            if grid > 0:
+                tiles = _xcd_layout(tiles, self.nproblems, grid=grid)
+            elif self.XCD_LAYOUT == "lpt" and len(tiles) >= 64:
                 tiles = _xcd_layout(tiles, self.nproblems)
             else:
                 # longest K first; ties keep problem order
@@ -525,7 +543,7 @@ class GemmProgram:
             d = _DeviceCache(probs=probs.reshape(-1), segs=self._segs.reshape(-1),
                              tiles=tiles.reshape(-1))
             d.ntiles = len(tiles)
-            self._by_tile[tile] = d
+            self._by_tile[key] = d
         return d.get(device), d.ntiles
 
     def run(self, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, precision: str | None = None,
@@ -543,13 +561,15 @@ class GemmProgram:
             raise ValueError("gemm: buffers smaller than the compiled plan")
         prec = precision or gemm_precision(C.dtype)
         tile = tuple(tile) if tile else self.tile
-        t, ntiles = self._tables(tile, C.device)
-        if grid < 0:
-            # persistent: every resident workgroup walks the tile list, prefetching the
-            # next tile's first K step while it finishes the current one.  Measured slower
-            # than one workgroup per tile on uneven sector sizes (static assignment loses
-            # the dispatcher's dynamic balancing), so it is opt-in.
-            grid = min(ntiles, resident_workgroups(prec, tile))
+        if grid < 0 and self.skinny is None:
+            # persistent: every resident workgroup walks its share of the tile list (dealt
+            # per XCD, longest first), prefetching the next tile's first K step while it
+            # finishes the current one
+            grid = resident_workgroups(prec, tile)
+            t, ntiles = self._tables(tile, C.device, grid)
+        else:
+            grid = max(grid, 0)
+            t, ntiles = self._tables(tile, C.device)
         if self.gather:
             W = C
         else:
