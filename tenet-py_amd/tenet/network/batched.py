@@ -1,11 +1,13 @@
 """Many independent two-site H_eff problems in one launch per step.
 
-``BatchedHeff2`` takes the compiled abelian chains of any number of bonds/chains (each a
-``_FusedChain``: GEMM, fused two-site MPO pass, transposed GEMM), packs every instance's
-operands into shared arena buffers and merges their tables with offset shifts, so one
-matvec of the whole batch is three launches: one persistent grouped GEMM over every
-instance's step-1 blocks, one MPO pass over every slab pair, one grouped GEMM over every
-step-3 block.  Uneven sector sizes of all instances share one tile table, sorted
+``BatchedHeff2`` takes the compiled abelian chains of any number of bonds/chains, packs
+every instance's operands into shared arena buffers and merges their tables with offset
+shifts, so one matvec of the whole batch is a fixed handful of launches whatever the
+batch size.  ``layout="sector"`` (default) merges ``_SectorChain`` tables: entry gather,
+one grouped GEMM over every instance's per-sector step-1 problems, one MPO pass, one
+grouped GEMM over every per-sector step-3 problem (K slices to a workspace) and the exit
+pass summing the slices into every instance's blocks.  ``layout="blocks"`` merges
+``_FusedChain`` tables (one problem per block pair, three launches).  Uneven sector sizes of all instances share one tile table, sorted
 longest-K first; the tile shape of each GEMM is chosen by the autotuner
 (``tenet.autotune``) from the histogram of its problem sizes.
 
@@ -101,9 +103,15 @@ def _shift_problems(problems, c_shift, a_shift, b_shift):
 class BatchedHeff2:
     """One matvec over a batch of abelian two-site H_eff instances."""
 
-    def __init__(self, instances, precision: str = "fp32", tiles: dict | None = None) -> None:
+    def __init__(self, instances, precision: str = "fp32", tiles: dict | None = None,
+                 layout: str = "sector") -> None:
         dt = torch.float64 if precision == "fp64" else torch.float32
-        self.precision, self.dtype = precision, dt
+        self.precision, self.dtype, self.layout = precision, dt, layout
+        if layout == "sector":
+            self._init_sector(instances, tiles or {})
+            return
+        if layout != "blocks":
+            raise ValueError(f"unknown layout {layout!r}")
         self.ops = [Heff2(fl, w1, w2, fr, precision, layout="blocks") for fl, w1, w2, fr, _ in instances]
         self.structures = [inst[4].structure for inst in instances]
         chains = [op.chain(s) for op, s in zip(self.ops, self.structures)]
@@ -155,6 +163,78 @@ class BatchedHeff2:
         self.flops = self.gemm1.flops + self.gemm4.flops
         self.count = len(instances)
 
+    def _init_sector(self, instances, tune: dict) -> None:
+        from tenet.engine import RecoupleProgram
+        from tenet.network.heff import exit_regions
+
+        self.ops = [Heff2(fl, w1, w2, fr, self.precision, layout="sector") for fl, w1, w2, fr, _ in instances]
+        self.structures = [inst[4].structure for inst in instances]
+        chains = [op.chain(s) for op, s in zip(self.ops, self.structures)]
+        for ch in chains:
+            if not hasattr(ch, "exit_spec"):
+                raise ValueError("BatchedHeff2 needs abelian (sector-path) operators")
+
+        def offsets(sizes):
+            o = np.zeros(len(sizes) + 1, dtype=np.int64)
+            np.cumsum(sizes, out=o[1:])
+            return o
+        self.x_off = offsets([st.size for st in self.structures])
+        xs_off = offsets([ch.xs_size for ch in chains])
+        frs_off = offsets([ch.frs_size for ch in chains])
+        fls_off = offsets([ch.fls_size for ch in chains])
+        t1_off = offsets([ch.t1_size for ch in chains])
+        t3_off = offsets([ch.t3_size for ch in chains])
+        p3_off = offsets([len(ch.problems3) for ch in chains])
+        self.x_size = int(self.x_off[-1])
+        self.xs_size, self.t1_size, self.t3_size = int(xs_off[-1]), int(t1_off[-1]), int(t3_off[-1])
+        # stacked environments of every instance, once
+        stacks = [ch.prepare(op._datas) for op, ch in zip(self.ops, chains)]
+        self.fr = torch.cat([a for a, _ in stacks])
+        self.fl = torch.cat([b for _, b in stacks])
+        entry, p1, p3 = [], [], []
+        jobs, ins, outs, coef, tl = [], [], [], [], []
+        for i, ch in enumerate(chains):
+            xo, so = int(self.x_off[i]), int(xs_off[i])
+            entry += [(d + so, shp, ds, ss, [(t + xo, c) for t, c in terms])
+                      for d, shp, ds, ss, terms in ch.xs_regions]
+            p1 += _shift_problems(ch.problems1, int(t1_off[i]), so, int(frs_off[i]))
+            p3 += _shift_problems(ch.problems3, 0, int(fls_off[i]), int(t3_off[i]))
+            i0, o0, c0 = len(ins), len(outs), len(coef)
+            cj, ci, co, cc = ch.mpo_host
+            for R, S, ib, ie, ob, oe, cf, _ in cj:
+                jobs.append((R, S, ib + i0, ie + i0, ob + o0, oe + o0, cf + c0, 0))
+                nt = -(-(R * S) // 1024)
+                tl.append(np.stack([np.full(nt, len(jobs) - 1, np.int64), np.arange(nt, dtype=np.int64)], 1))
+            ins += [(off + int(t1_off[i]), rs) for off, rs in ci]
+            outs += [(off + int(t3_off[i]), rs) for off, rs in co]
+            coef += cc
+        self.entry = RecoupleProgram(entry, self.x_size, max(self.xs_size, 1))
+        self.gemm1 = GemmProgram(p1, max(self.xs_size, 1), int(frs_off[-1]) or 1, max(self.t1_size, 1),
+                                 tile=tune.get("gemm1") or (64, 64))
+        self.gemm4 = GemmProgram(p3, int(fls_off[-1]) or 1, max(self.t3_size, 1), 1, trans_a=True,
+                                 tile=tune.get("gemm4") or (64, 64), gather=True)
+        ex = []
+        for i, ch in enumerate(chains):
+            ex += exit_regions(ch.exit_spec, self.gemm4.slices, int(self.x_off[i]), int(p3_off[i]))
+        self.exit = RecoupleProgram(ex, max(self.gemm4.ws_size, 1), self.x_size)
+        tl = np.concatenate(tl) if tl else np.zeros((0, 2), np.int64)
+        self.mpo_njobs, self.mpo_ntiles = len(jobs), len(tl)
+        self.mpo = _DeviceCache(jobs=np.asarray(jobs, np.int64).reshape(-1),
+                                ins=np.asarray(ins, np.int64).reshape(-1),
+                                outs=np.asarray(outs, np.int64).reshape(-1),
+                                coef=np.asarray(coef, np.float64 if self.dtype == torch.float64 else np.float32),
+                                tiles=tl.reshape(-1))
+        self.flops = self.gemm1.flops + self.gemm4.flops
+        self.count = len(instances)
+
+    def _mpo(self, t1: torch.Tensor, t3: torch.Tensor) -> None:
+        tb = self.mpo.get(t1.device)
+        _native.check(_native.lib().tn_mpo_apply(
+            _native.TN_F64 if t1.dtype == torch.float64 else _native.TN_F32,
+            t1.data_ptr(), t3.data_ptr(), tb["jobs"].data_ptr(), self.mpo_njobs,
+            tb["ins"].data_ptr(), tb["outs"].data_ptr(), tb["coef"].data_ptr(),
+            tb["tiles"].data_ptr(), self.mpo_ntiles, _native.stream_ptr()))
+
     def pack(self, vectors) -> torch.Tensor:
         """One arena buffer (working dtype) holding every instance's input vector."""
         return torch.cat([v.data.to(self.dtype) for v in vectors])
@@ -166,15 +246,22 @@ class BatchedHeff2:
         if x.dtype != self.dtype or x.numel() != self.x_size:
             raise ValueError("BatchedHeff2.apply: the arena does not match the batch")
         t = tiles or {}
+        if self.layout == "sector":
+            xs = torch.empty(max(self.xs_size, 1), dtype=x.dtype, device=x.device)
+            self.entry.run(x, xs)
+            t1 = torch.empty(max(self.t1_size, 1), dtype=x.dtype, device=x.device)
+            self.gemm1.run(xs, self.fr, t1, self.precision, tile=t.get("gemm1"))
+            t3 = torch.empty(max(self.t3_size, 1), dtype=x.dtype, device=x.device)
+            self._mpo(t1, t3)
+            ws = torch.empty(max(self.gemm4.ws_size, 1), dtype=x.dtype, device=x.device)
+            self.gemm4.run(self.fl, t3, ws, self.precision, tile=t.get("gemm4"))
+            out = torch.empty(self.x_size, dtype=x.dtype, device=x.device)
+            self.exit.run(ws, out)
+            return out
         t1 = torch.empty(self.t1_size, dtype=x.dtype, device=x.device)
         self.gemm1.run(x, self.fr, t1, self.precision, tile=t.get("gemm1"))
         t3 = torch.empty(self.t3_size, dtype=x.dtype, device=x.device)
-        tb = self.mpo.get(x.device)
-        _native.check(_native.lib().tn_mpo_apply(
-            _native.TN_F64 if x.dtype == torch.float64 else _native.TN_F32,
-            t1.data_ptr(), t3.data_ptr(), tb["jobs"].data_ptr(), self.mpo_njobs,
-            tb["ins"].data_ptr(), tb["outs"].data_ptr(), tb["coef"].data_ptr(),
-            tb["tiles"].data_ptr(), self.mpo_ntiles, _native.stream_ptr()))
+        self._mpo(t1, t3)
         out = torch.empty(self.x_size, dtype=x.dtype, device=x.device)
         self.gemm4.run(self.fl, t3, out, self.precision, tile=t.get("gemm4"))
         return out

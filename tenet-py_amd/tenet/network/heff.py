@@ -398,6 +398,13 @@ def _mpo_jobs(inputs, outputs, w12, src_size, dst_size):
     return jobs, ins, outs, coef, tiles
 
 
+def exit_regions(spec, slices, dst_shift: int = 0, problem_shift: int = 0):
+    """Recouple regions of a sector chain's exit pass: every output block is the sum of the
+    K slices of its step-3 problem, read at its column offset."""
+    return [(d + dst_shift, shape, dstr, sstr, [(o + col, 1.0) for o in slices[p + problem_shift]])
+            for d, shape, dstr, sstr, p, col in spec]
+
+
 class _SectorChain:
     """Abelian fast path on charge-sector stacked operands.
 
@@ -470,6 +477,8 @@ class _SectorChain:
             problems1.append((m, n_r[r], t1_acc, n_r[r], [(xs_base[r], frs_base[r], dr, dr, n_r[r])]))
             t1_acc += m * n_r[r]
         self.xs_size, self.frs_size, self.t1_size = acc, facc, t1_acc
+        # host specs kept beside the compiled programs: BatchedHeff2 merges them across bonds
+        self.xs_regions, self.frs_regions, self.problems1 = xs_regions, frs_regions, problems1
         self.entry = RecoupleProgram(xs_regions, s_in.size, max(acc, 1))
         self.fr_stack = RecoupleProgram(frs_regions, fr_s.size, max(facc, 1))
         # 64 x 64 workgroup tiles: the stacked sectors are few and uneven, and larger tiles
@@ -511,6 +520,7 @@ class _SectorChain:
             problems3.append((db, c0, 0, c0, [(fls_base[B], t3_base[B], kb, db, c0)] if kb else []))
             order3.append(B)
         self.fls_size, self.t3_size = lacc, t3_acc
+        self.fls_regions, self.problems3 = fls_regions, problems3
         self.fl_stack = RecoupleProgram(fls_regions, fl_s.size, max(lacc, 1))
         self.gemm4 = GemmProgram(problems3, max(lacc, 1), max(t3_acc, 1), 1, trans_a=True, tile=(64, 64),
                                  gather=True)
@@ -564,15 +574,16 @@ class _SectorChain:
         )
         self.mpo_bytes_elems = t1_acc + t3_acc
         # ---- exit: sum of the K slices of out_B, scattered into aa's blocks ----
-        ex_regions = []
-        for B, sl in zip(order3, self.gemm4.slices):
+        # spec per output block: (dst_off, shape, dst_strides, src_strides, problem, column)
+        self.exit_spec = []
+        for p, B in enumerate(order3):
             N = n_b[B]
             for j in out_cols[B]:
                 db, dP, dQ, ds = (int(v) for v in L.shapes[j])
                 n = dP * dQ * ds
-                ex_regions.append((int(L.offsets[j]), (db, n), (n, 1), (N, 1),
-                                   [(o + out_coloff[j], 1.0) for o in sl]))
-        self.exit = RecoupleProgram(ex_regions, max(self.gemm4.ws_size, 1), s_in.size)
+                self.exit_spec.append((int(L.offsets[j]), (db, n), (n, 1), (N, 1), p, out_coloff[j]))
+        self.exit = RecoupleProgram(exit_regions(self.exit_spec, self.gemm4.slices),
+                                    max(self.gemm4.ws_size, 1), s_in.size)
         self.flops = self.gemm1.flops + self.gemm4.flops
 
     def prepare(self, datas) -> tuple:

@@ -103,13 +103,18 @@ def test_dmrg_xx_chain_matches_closed_form():
     assert abs(out.energy - ed.xx_energy(L)) < 1e-10 * abs(ed.xx_energy(L))
 
 
+@pytest.mark.parametrize("layout", ["sector", "blocks"])
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
-def test_batched_heff2_matches_individual_operators(prec, tol):
+def test_batched_heff2_matches_individual_operators(prec, tol, layout):
     from tenet.network import Heff2
     from tenet.network.batched import BatchedHeff2, synthetic_operator
 
-    insts = [synthetic_operator(16, d, 3.0, dl, s, device="cuda") for d, dl, s in ((40, 0.5, 1), (70, 1.3, 2), (24, 1.9, 3))]
-    b = BatchedHeff2(insts, prec)
+    # the 24-site D=600 instance makes the merged step-3 GEMM split its K into slices
+    insts = [synthetic_operator(n, d, sg, dl, s, device="cuda")
+             for n, d, sg, dl, s in ((16, 40, 3.0, 0.5, 1), (24, 600, 4.0, 1.3, 2), (16, 24, 3.0, 1.9, 3))]
+    b = BatchedHeff2(insts, prec, layout=layout)
+    if layout == "sector":
+        assert b.gemm4.ws_size > b.x_size
     out = b.unpack(b.apply(b.pack([i[4] for i in insts])))
     for inst, got in zip(insts, out):
         ref = Heff2(*inst[:4], "fp64")(inst[4])
