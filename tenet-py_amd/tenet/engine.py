@@ -18,6 +18,7 @@ structures it was built from.
 
 from __future__ import annotations
 
+import math
 import os
 from collections import defaultdict
 from dataclasses import dataclass
@@ -134,7 +135,7 @@ class RecoupleProgram:
             if len(shp) > MAX_RANK:
                 raise ValueError(f"recouple region of rank {len(shp)} exceeds {MAX_RANK}")
             r = len(shp)
-            nelem = int(np.prod(shp))
+            nelem = math.prod(shp)
             mode = -1
             if ss[-1] != 1 and ds[-1] == 1 and shp[-1] >= 16:
                 cands = [k for k in range(r - 1) if ss[k] == 1 and shp[k] >= 16]
@@ -151,14 +152,14 @@ class RecoupleProgram:
             jobs[j, 23] = nelem
             for so, c in terms:
                 tsrc.append(so)
-                tco.append(float(np.real(c)))
+                tco.append(float(c.real) if isinstance(c, complex) else float(c))
             tb += len(terms)
             if mode < 0:
                 nt = -(-nelem // _LIN_TILE)
             else:
                 batch = nelem // (shp[-1] * shp[mode])
                 nt = (-(-shp[-1] // _LDS_T)) * (-(-shp[mode] // _LDS_T)) * batch
-            tiles.append((j, nt, mode, shp))
+            tiles.append((j, nt, mode, shp, nelem))
             self.elements += nelem
             self.moved_terms += nelem * len(terms)
             # every access stays in bounds: checked here once, on the host
@@ -173,18 +174,23 @@ class RecoupleProgram:
         # a program of few large regions would leave most of the device idle: give its LDS
         # jobs 32 x 32 tiles, four times the workgroups
         # (mode word J + 8); its linear jobs likewise get 512-element tiles (mode word -2)
-        small = sum(nt for _, nt, _, _ in tiles) < _SMALL_PROGRAM_TILES
-        tl = []
-        for j, nt, mode, shp in tiles:
+        small = sum(t[1] for t in tiles) < _SMALL_PROGRAM_TILES
+        counts = np.zeros(n, dtype=np.int64)
+        for j, nt, mode, shp, nelem in tiles:
             if mode >= 0 and small:
                 jobs[j, 22] = mode + 8
                 e = _LDS_T // 2
-                nt = (-(-shp[-1] // e)) * (-(-shp[mode] // e)) * (int(np.prod(shp)) // (shp[-1] * shp[mode]))
+                nt = (-(-shp[-1] // e)) * (-(-shp[mode] // e)) * (nelem // (shp[-1] * shp[mode]))
             elif mode < 0 and small:
                 jobs[j, 22] = -2
-                nt = -(-int(np.prod(shp)) // (_LIN_TILE // 4))
-            tl.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
-        tiles_arr = np.concatenate(tl) if tl else np.zeros((0, 2), np.int64)
+                nt = -(-nelem // (_LIN_TILE // 4))
+            counts[j] = nt
+        # tile table: (job, tile index within the job) for every job's tiles, in job order
+        total = int(counts.sum())
+        starts = np.cumsum(counts) - counts
+        tiles_arr = np.empty((total, 2), dtype=np.int64)
+        tiles_arr[:, 0] = np.repeat(np.arange(n, dtype=np.int64), counts)
+        tiles_arr[:, 1] = np.arange(total, dtype=np.int64) - np.repeat(starts, counts)
         self.ntiles = len(tiles_arr)
         self.src_size, self.dst_size = src_size, dst_size
         self._tables = _DeviceCache(
@@ -517,15 +523,17 @@ class GemmProgram:
             self._by_tile[tile] = d
         if d is None:
             bm, bn = tile
-            rows = []
-            for p, (M, N) in enumerate(zip(self.Ms, self.Ns)):
-                tm, tn = -(-M // bm), -(-N // bn)
-                if tm * tn == 0:
-                    continue
-                ii, jj = np.meshgrid(np.arange(tm), np.arange(tn), indexing="ij")
-                rows.append(np.stack([np.full(tm * tn, p), ii.ravel(), jj.ravel(),
-                                      np.full(tm * tn, self.Ks[p])], 1))
-            tiles = np.concatenate(rows) if rows else np.zeros((0, 4), np.int64)
+            Ms, Ns = np.asarray(self.Ms, np.int64), np.asarray(self.Ns, np.int64)
+            tm, tn = -(-Ms // bm), -(-Ns // bn)
+            cnt = tm * tn
+            total = int(cnt.sum())
+            local = np.arange(total, dtype=np.int64) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+            tn_rep = np.repeat(tn, cnt)
+            tiles = np.empty((total, 4), dtype=np.int64)
+            tiles[:, 0] = np.repeat(np.arange(len(Ms), dtype=np.int64), cnt)
+            tiles[:, 1] = local // np.maximum(tn_rep, 1)
+            tiles[:, 2] = local % np.maximum(tn_rep, 1)
+            tiles[:, 3] = np.repeat(np.asarray(self.Ks, np.int64), cnt)
             if grid > 0:
                 tiles = _xcd_layout(tiles, self.nproblems, grid=grid)
             elif self.XCD_LAYOUT == "lpt" and len(tiles) >= 64:

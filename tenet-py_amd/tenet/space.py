@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import enum
 from collections.abc import Hashable, Iterable, Mapping
-from dataclasses import dataclass, replace
+from dataclasses import dataclass, field, replace
 
 __all__ = ["GradedSpace", "Leg", "Side", "IN", "OUT"]
 
@@ -24,6 +24,16 @@ class GradedSpace:
 
     provider: object
     sectors: tuple
+    # derived once: plan caches hash spaces constantly, and degeneracy lookups are hot
+    _hash: int = field(default=0, init=False, repr=False, compare=False)
+    _map: dict = field(default=None, init=False, repr=False, compare=False)
+
+    def __post_init__(self) -> None:
+        object.__setattr__(self, "_hash", hash((self.provider, self.sectors)))
+        object.__setattr__(self, "_map", dict(self.sectors))
+
+    def __hash__(self) -> int:
+        return self._hash
 
     @classmethod
     def new(cls, provider, sectors: Mapping | Iterable) -> "GradedSpace":
@@ -46,14 +56,10 @@ class GradedSpace:
 
     def degeneracy(self, a) -> int:
         """``m_a``, or 0 when ``a`` is absent."""
-        return self._lookup().get(a, 0)
+        return self._map.get(a, 0)
 
     def _lookup(self) -> dict:
-        try:
-            return _LOOKUP[self]
-        except KeyError:
-            d = _LOOKUP[self] = dict(self.sectors)
-            return d
+        return self._map
 
     def __contains__(self, a) -> bool:
         return self.degeneracy(a) > 0
@@ -77,9 +83,6 @@ class GradedSpace:
     def flat(self) -> "GradedSpace":
         """Every degeneracy set to 1: the part of a space a block set depends on."""
         return GradedSpace(self.provider, tuple((a, 1) for a, _ in self.sectors))
-
-
-_LOOKUP: dict = {}
 
 
 class Side(enum.Enum):

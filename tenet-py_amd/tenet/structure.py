@@ -188,9 +188,8 @@ def _axis_sectors(s: TensorStructure) -> tuple:
 
 @cache
 def _shapes(s: TensorStructure) -> tuple:
-    return tuple(
-        tuple(leg.degeneracy(a) for leg, a in zip(s.legs, secs)) for secs in _axis_sectors(s)
-    )
+    maps = [leg.space._map for leg in s.legs]  # sector -> degeneracy, per axis
+    return tuple(tuple(m.get(a, 0) for m, a in zip(maps, secs)) for secs in _axis_sectors(s))
 
 
 @dataclass(frozen=True)
