@@ -13,13 +13,24 @@ any mix.
 
 ``poisson_requests`` draws the synthetic request stream; ``serve`` runs it and returns the
 per-request results; ``latency_report`` gives p50/p99 time-to-converged and solves/hour.
+
+A solve's per-bond host work (planning, Python orchestration, the keep-rule readback) is
+larger than its kernels at these sizes, so one process leaves the GPU idle most of the
+time.  ``MultiProcessServer`` runs ``workers`` such servers as separate processes on the
+same GPU (each with its own streams and HBM budget share): a dispatcher releases every
+request into a shared queue at its arrival time and the first worker with a free slot takes
+it.  Each solve runs the same kernels on the same inputs as in the single-process server,
+so its energy agrees with that server's to rounding.
 """
 
 from __future__ import annotations
 
 import math
+import multiprocessing as mp
+import queue as queue_mod
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
+from functools import partial
 
 import numpy as np
 import torch
@@ -28,7 +39,7 @@ from tenet.network.dmrg import dmrg_steps
 from tenet.network.models import heisenberg_mpo, random_chain
 
 __all__ = ["SolveRequest", "SolveResult", "DMRGServer", "poisson_requests", "latency_report",
-           "footprint_bytes"]
+           "footprint_bytes", "MultiProcessServer"]
 
 
 @dataclass
@@ -99,8 +110,21 @@ class DMRGServer:
                              precision=self.precision)
         return _Active(req, gen, stream, now, footprint_bytes(req))
 
-    def serve(self, requests, *, clock=time.perf_counter) -> list:
-        """Run a request stream to completion; arrival times are relative to the start."""
+    def _step(self, a: _Active, clock) -> SolveResult | None:
+        """One bond step of ``a``; its result once the solve has converged."""
+        with torch.cuda.stream(a.stream):
+            try:
+                next(a.gen)
+                return None
+            except StopIteration as stop:
+                out = stop.value
+                a.stream.synchronize()
+                return SolveResult(a.req.rid, out.energy, out.sweeps, list(out.history),
+                                   a.req.arrival, a.admitted, clock())
+
+    def serve(self, requests, *, clock=time.perf_counter, on_result=None) -> list:
+        """Run a request stream to completion; arrival times are relative to the start.
+        ``on_result`` is called with each ``SolveResult`` as its solve converges."""
         queue = sorted(requests, key=lambda r: (r.arrival, r.rid))
         t0 = clock()
         active: list = []
@@ -123,18 +147,151 @@ class DMRGServer:
                 continue
             # one bond step of every active solve, round robin
             for a in list(active):
-                with torch.cuda.stream(a.stream):
-                    try:
-                        next(a.gen)
-                    except StopIteration as stop:
-                        out = stop.value
-                        a.stream.synchronize()
-                        done.append(SolveResult(a.req.rid, out.energy, out.sweeps, out.history,
-                                                a.req.arrival, a.admitted, clock() - t0))
-                        active.remove(a)
-                        free_streams.append(a.stream)
-                        used -= a.bytes
+                res = self._step(a, lambda: clock() - t0)
+                if res is not None:
+                    done.append(res)
+                    if on_result is not None:
+                        on_result(res)
+                    active.remove(a)
+                    free_streams.append(a.stream)
+                    used -= a.bytes
         return sorted(done, key=lambda r: r.rid)
+
+
+def _gpu_server(max_active: int, precision: str, share: float) -> DMRGServer:
+    """A worker's server: ``share`` of the HBM free when it starts."""
+    free, _ = torch.cuda.mem_get_info()
+    return DMRGServer(memory_budget=int(0.8 * free * share), max_active=max_active, precision=precision)
+
+
+def _worker_main(wid: int, req_q, res_q, factory) -> None:
+    """One serving process: continuous batching over the requests it takes from ``req_q``
+    (``None``: no more requests).  A taken request that does not fit the memory budget
+    beside the running solves is held until it does.  Time stamps are wall-clock
+    (``time.time``); the dispatcher makes them relative."""
+    try:
+        server = factory()
+    except BaseException as exc:
+        res_q.put(("error", wid, f"failed to start: {exc!r}"))
+        raise
+    res_q.put(("ready", wid, None))
+    try:
+        _worker_loop(wid, server, req_q, res_q)
+    except BaseException as exc:
+        res_q.put(("error", wid, repr(exc)))
+        raise
+    res_q.put(("done", wid, None))
+
+
+def _worker_loop(wid: int, server, req_q, res_q) -> None:
+    active, free_streams, used = [], list(server.streams), 0
+    held, closing = None, False
+    while True:
+        while free_streams and not closing:
+            if held is None:
+                try:
+                    item = req_q.get_nowait() if active else req_q.get(timeout=0.05)
+                except queue_mod.Empty:
+                    break
+                if item is None:
+                    closing = True
+                    break
+                held = item
+            if active and used + footprint_bytes(held) > server.memory_budget:
+                break
+            a = server._start(held, free_streams.pop(), time.time())
+            held = None
+            used += a.bytes
+            active.append(a)
+        if not active:
+            if closing and held is None:
+                break
+            continue
+        for a in list(active):
+            res = server._step(a, time.time)
+            if res is not None:
+                res_q.put(("result", wid, res))
+                active.remove(a)
+                free_streams.append(a.stream)
+                used -= a.bytes
+
+
+class MultiProcessServer:
+    """``workers`` serving processes on the one GPU with ``max_active`` solves in flight in
+    total (``max_active // workers`` each) and an equal share of the free HBM each.
+
+    Workers are spawned (a fresh interpreter each, never a fork of this process) and boot
+    before the request stream's clock starts; ``boot_s`` is how long that took.  ``serve``
+    releases each request into the shared queue at its arrival time; the first worker with a
+    free slot takes it.  ``factory`` (a picklable callable returning an object with the
+    ``DMRGServer`` worker interface) replaces the GPU server in host-only tests."""
+
+    def __init__(self, *, workers: int = 4, max_active: int = 8, precision: str = "fp32",
+                 factory=None) -> None:
+        if workers < 1 or max_active < workers:
+            raise ValueError(f"need 1 <= workers <= max_active, got {workers}, {max_active}")
+        self.workers = workers
+        self.factory = factory or partial(_gpu_server, max_active // workers, precision, 1.0 / workers)
+        self.boot_s = None
+
+    def serve(self, requests, *, boot_timeout: float = 600.0, on_result=None) -> list:
+        ctx = mp.get_context("spawn")
+        req_q, res_q = ctx.Queue(), ctx.Queue()
+        procs = [ctx.Process(target=_worker_main, args=(w, req_q, res_q, self.factory), daemon=True)
+                 for w in range(self.workers)]
+        tb = time.time()
+        for p in procs:
+            p.start()
+        try:
+            ready = 0
+            while ready < len(procs):
+                try:
+                    tag, wid, val = res_q.get(timeout=1.0)
+                except queue_mod.Empty:
+                    dead = [w for w, p in enumerate(procs) if p.exitcode is not None]
+                    if dead:
+                        raise RuntimeError(f"serving worker(s) {dead} exited while starting "
+                                           f"(exit codes {[procs[w].exitcode for w in dead]})")
+                    if time.time() - tb > boot_timeout:
+                        raise RuntimeError(f"serving workers not ready after {boot_timeout:.0f} s")
+                    continue
+                if tag != "ready":
+                    raise RuntimeError(f"serving worker {wid}: {val}")
+                ready += 1
+            t0 = time.time()
+            self.boot_s = t0 - tb
+            pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+            results, qi, closed = [], 0, False
+            while len(results) < len(pending):
+                now = time.time() - t0
+                while qi < len(pending) and pending[qi].arrival <= now:
+                    req_q.put(pending[qi])
+                    qi += 1
+                if qi == len(pending) and not closed:
+                    for _ in procs:
+                        req_q.put(None)  # after every request: a worker that takes it has no more to do
+                    closed = True
+                wait = pending[qi].arrival - now if qi < len(pending) else 1.0
+                try:
+                    tag, wid, val = res_q.get(timeout=max(0.001, min(wait, 1.0)))
+                except queue_mod.Empty:
+                    if not any(p.is_alive() for p in procs):
+                        raise RuntimeError("every serving worker exited before the stream finished")
+                    continue
+                if tag == "error":
+                    raise RuntimeError(f"serving worker {wid} failed: {val}")
+                if tag == "result":
+                    results.append(replace(val, admitted=val.admitted - t0, finished=val.finished - t0))
+                    if on_result is not None:
+                        on_result(results[-1])
+            for p in procs:
+                p.join(timeout=60)
+            return sorted(results, key=lambda r: r.rid)
+        finally:
+            for p in procs:
+                if p.is_alive():
+                    p.terminate()
+                    p.join(timeout=10)
 
 
 def poisson_requests(count: int, rate: float, *, sizes=(32, 64, 128), chis=(128, 256, 512, 1024),

@@ -40,3 +40,16 @@ def test_admission_respects_the_memory_budget():
     spans = sorted((r.admitted, r.finished) for r in res)
     for (a0, f0), (a1, f1) in zip(spans, spans[1:]):
         assert a1 >= f0 - 1e-9  # never two solves resident together
+
+
+def test_multiprocess_server_matches_the_in_process_server():
+    from tenet.network.serve import DMRGServer, MultiProcessServer
+
+    srv = MultiProcessServer(workers=2, max_active=4)
+    got = srv.serve(_reqs(), boot_timeout=300)
+    ref = DMRGServer(max_active=3).serve(_reqs())
+    assert [r.rid for r in got] == [r.rid for r in ref]
+    for g, r in zip(got, ref):
+        assert g.sweeps == r.sweeps
+        assert abs(g.energy - r.energy) <= 1e-12 * abs(r.energy)
+        assert g.admitted >= g.arrival - 1e-3 and g.finished > g.admitted
