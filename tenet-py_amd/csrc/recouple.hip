@@ -20,18 +20,11 @@ constexpr int kLinPerThread = 8;
 constexpr int kLinTile = kThreads * kLinPerThread;
 constexpr int kT = 64;  // LDS tile edge
 
-template <typename T, typename TS>
-__device__ __forceinline__ T gather_terms(const TS* __restrict__ src, const int64_t* __restrict__ tsrc,
-                                          const double* __restrict__ tco, int64_t tb, int64_t te,
-                                          int64_t soff) {
-  T acc = T(0);
-  for (int64_t t = tb; t < te; ++t) acc += static_cast<T>(tco[t]) * static_cast<T>(src[tsrc[t] + soff]);
-  return acc;
-}
-
 // Linear tile: element e of the job in destination C order, PER elements per thread.  All
-// of a thread's elements are decoded first and their loads issued together (independent,
-// one round of memory latency per term instead of one per element).
+// of a thread's elements are decoded first and their loads issued unconditionally (a lane
+// past the region's end reads the region's first element, in bounds by the host's checks,
+// and discards it): a load under a per-lane branch would be waited for before the next one
+// is issued, one memory round trip per element instead of one per term.
 template <typename T, typename TS, typename TD, int PER>
 __device__ __forceinline__ void linear_tile(const TS* __restrict__ src, TD* __restrict__ dst,
                                             const int64_t* __restrict__ tsrc,
@@ -47,17 +40,19 @@ __device__ __forceinline__ void linear_tile(const TS* __restrict__ src, TD* __re
   for (int r = 0; r < PER; ++r) {
     const int64_t e = base + r * kThreads + tid;
     live[r] = e < nelem;
-    so[r] = 0;
-    dof[r] = 0;
     uint32_t rem = live[r] ? static_cast<uint32_t>(e) : 0u;
-    for (int d = rank - 1; d >= 0; --d) {
-      const uint32_t n = static_cast<uint32_t>(shape[d]);
+    int64_t s = 0, d = 0;
+    for (int k = rank - 1; k > 0; --k) {
+      const uint32_t n = static_cast<uint32_t>(shape[k]);
       const uint32_t q = rem / n;
       const int64_t i = rem - q * n;
       rem = q;
-      so[r] += i * sstr[d];
-      dof[r] += i * dstr[d];
+      s += i * sstr[k];
+      d += i * dstr[k];
     }
+    // what is left indexes dim 0 (e < nelem bounds it by shape[0])
+    so[r] = s + static_cast<int64_t>(rem) * sstr[0];
+    dof[r] = d + static_cast<int64_t>(rem) * dstr[0];
   }
   T acc[PER];
 #pragma unroll
@@ -65,13 +60,87 @@ __device__ __forceinline__ void linear_tile(const TS* __restrict__ src, TD* __re
   for (int64_t t = tb; t < te; ++t) {
     const T c = static_cast<T>(tco[t]);
     const TS* __restrict__ s0 = src + tsrc[t];
+    T v[PER];
 #pragma unroll
-    for (int r = 0; r < PER; ++r)
-      if (live[r]) acc[r] += c * static_cast<T>(s0[so[r]]);
+    for (int r = 0; r < PER; ++r) v[r] = static_cast<T>(s0[so[r]]);
+#pragma unroll
+    for (int r = 0; r < PER; ++r) acc[r] += c * v[r];
   }
 #pragma unroll
   for (int r = 0; r < PER; ++r)
     if (live[r]) dst[dst_off + dof[r]] = static_cast<TD>(acc[r]);
+}
+
+// LDS transpose tile of edge E over (I = rank-1, J); remaining dims are batch.  Loads run
+// along J (source-contiguous), all of a thread's loads of one term issued together (lanes
+// outside the tile read its first element and drop it); stores run along I
+// (destination-contiguous).
+template <typename T, typename TS, typename TD, int E>
+__device__ __forceinline__ void lds_tile(T (*lds)[kT + 1], const TS* __restrict__ src, TD* __restrict__ dst,
+                                         const int64_t* __restrict__ tsrc, const double* __restrict__ tco,
+                                         int64_t tb, int64_t te, int rank, int J, const int64_t* shape,
+                                         const int64_t* dstr, const int64_t* sstr, int64_t dst_off,
+                                         int64_t tile) {
+  constexpr int R = E * E / kThreads;
+  const int tid = threadIdx.x;
+  const int I = rank - 1;
+  const int64_t nI = (shape[I] + E - 1) / E;
+  const int64_t nJ = (shape[J] + E - 1) / E;
+  const int64_t ti = tile % nI;
+  const int64_t tj = (tile / nI) % nJ;
+  uint32_t b = static_cast<uint32_t>(tile / (nI * nJ));
+  int64_t sbase = 0, dbase = 0;
+  for (int d = rank - 1; d >= 0; --d) {
+    if (d == I || d == J) continue;
+    const uint32_t n = static_cast<uint32_t>(shape[d]);
+    const uint32_t q = b / n;
+    const int64_t i = b - q * n;
+    b = q;
+    sbase += i * sstr[d];
+    dbase += i * dstr[d];
+  }
+  const int64_t i0 = ti * E, j0 = tj * E;
+  const int64_t ni = min<int64_t>(E, shape[I] - i0);
+  const int64_t nj = min<int64_t>(E, shape[J] - j0);
+  const int64_t sI = sstr[I], sJ = sstr[J], dJ = dstr[J], dI = dstr[I];
+  const int64_t first = sbase + i0 * sI + j0 * sJ;
+  // C loads in flight per thread at a time: enough to cover the latency, few registers
+  constexpr int C = R < 8 ? R : 8;
+#pragma unroll
+  for (int r0 = 0; r0 < R; r0 += C) {
+    int64_t off[C];
+    bool ok[C];
+    T v[C];
+#pragma unroll
+    for (int r = 0; r < C; ++r) {
+      const int idx = (r0 + r) * kThreads + tid;
+      const int j = idx % E, i = idx / E;
+      ok[r] = i < ni && j < nj;
+      off[r] = ok[r] ? first + i * sI + j * sJ : first;
+      v[r] = T(0);
+    }
+    for (int64_t t = tb; t < te; ++t) {
+      const T c = static_cast<T>(tco[t]);
+      const TS* __restrict__ s0 = src + tsrc[t];
+      T x[C];
+#pragma unroll
+      for (int r = 0; r < C; ++r) x[r] = static_cast<T>(s0[off[r]]);
+#pragma unroll
+      for (int r = 0; r < C; ++r) v[r] += c * x[r];
+    }
+#pragma unroll
+    for (int r = 0; r < C; ++r) {
+      const int idx = (r0 + r) * kThreads + tid;
+      if (ok[r]) lds[idx / E][idx % E] = v[r];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int idx = r * kThreads + tid;
+    const int i = idx % E, j = idx / E;
+    if (i < ni && j < nj) dst[dst_off + dbase + (j0 + j) * dJ + (i0 + i) * dI] = static_cast<TD>(lds[i][j]);
+  }
 }
 
 // TS: source element type, TD: destination element type (a cast rides along with the
@@ -95,7 +164,6 @@ __global__ __launch_bounds__(kThreads) void recouple_kernel(const TS* __restrict
   const int64_t* sstr = jw + 16;
   const int64_t mode = jw[22];
   const int64_t nelem = jw[23];
-  const int tid = threadIdx.x;
 
   if (mode < 0) {
     // linear tiles: 2048 elements (mode -1) or, for programs with few elements, 512 (mode -2)
@@ -107,44 +175,14 @@ __global__ __launch_bounds__(kThreads) void recouple_kernel(const TS* __restrict
     return;
   }
 
-  // LDS transpose tile over (I = rank-1, J = mode); remaining dims are batch.  Mode J + 8
-  // asks for a 32x32 tile (four times the workgroups for a program with few large regions).
-  const int E = mode >= 8 ? kT / 2 : kT;
-  const int I = rank - 1;
-  const int J = static_cast<int>(mode >= 8 ? mode - 8 : mode);
-  const int64_t nI = (shape[I] + E - 1) / E;
-  const int64_t nJ = (shape[J] + E - 1) / E;
-  const int64_t ti = tile % nI;
-  const int64_t tj = (tile / nI) % nJ;
-  uint32_t b = static_cast<uint32_t>(tile / (nI * nJ));
-  int64_t sbase = 0, dbase = 0;
-  for (int d = rank - 1; d >= 0; --d) {
-    if (d == I || d == J) continue;
-    const uint32_t n = static_cast<uint32_t>(shape[d]);
-    const uint32_t q = b / n;
-    const int64_t i = b - q * n;
-    b = q;
-    sbase += i * sstr[d];
-    dbase += i * dstr[d];
-  }
-  const int64_t i0 = ti * E, j0 = tj * E;
-  const int64_t ni = min<int64_t>(E, shape[I] - i0);
-  const int64_t nj = min<int64_t>(E, shape[J] - j0);
-  const int64_t sI = sstr[I], sJ = sstr[J], dJ = dstr[J], dI = dstr[I];
-  // load: threads run along J (source-contiguous)
-  for (int r = 0; r < E * E / kThreads; ++r) {
-    const int idx = r * kThreads + tid;
-    const int j = idx % E, i = idx / E;
-    if (i < ni && j < nj)
-      lds[i][j] = gather_terms<T>(src, tsrc, tco, tb, te, sbase + (i0 + i) * sI + (j0 + j) * sJ);
-  }
-  __syncthreads();
-  // store: threads run along I (destination-contiguous)
-  for (int r = 0; r < E * E / kThreads; ++r) {
-    const int idx = r * kThreads + tid;
-    const int i = idx % E, j = idx / E;
-    if (i < ni && j < nj) dst[dst_off + dbase + (j0 + j) * dJ + (i0 + i) * dI] = static_cast<TD>(lds[i][j]);
-  }
+  // LDS transpose tile over (I = rank-1, J = mode); mode J + 8 asks for a 32x32 tile (four
+  // times the workgroups for a program with few large regions)
+  if (mode >= 8)
+    lds_tile<T, TS, TD, kT / 2>(lds, src, dst, tsrc, tco, tb, te, rank, static_cast<int>(mode - 8), shape, dstr,
+                                sstr, dst_off, tile);
+  else
+    lds_tile<T, TS, TD, kT>(lds, src, dst, tsrc, tco, tb, te, rank, static_cast<int>(mode), shape, dstr, sstr,
+                            dst_off, tile);
 }
 
 }  // namespace
