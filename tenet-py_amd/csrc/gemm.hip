@@ -462,6 +462,43 @@ int launch_prec(int ta, int tb, int bm, int bn, const void* A, const void* B, vo
 // 2 int32 per tile: (problem, block of 256 lines).
 constexpr int kSkinnyMax = 8;
 
+// One line (output row or column) of a skinny problem with exactly NW <= 8 entries across:
+// every load of a k step is issued unconditionally (a load under a per-entry branch is
+// waited for before the next one is issued), four k steps unrolled.
+template <typename T, bool TA, bool TB, bool ROWS, int NW>
+__device__ __forceinline__ void skinny_line(const T* __restrict__ A, const T* __restrict__ B, T* __restrict__ out,
+                                            const int64_t* __restrict__ pw, const int64_t* __restrict__ segs,
+                                            int64_t line, int64_t c_off, int64_t ldc) {
+  T acc[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) acc[j] = T(0);
+  for (int64_t s = pw[4]; s < pw[5]; ++s) {
+    const int64_t* sw = segs + s * TN_SEG_WORDS;
+    const T* a = A + sw[0];
+    const T* b = B + sw[1];
+    const int64_t K = sw[2], lda = sw[3], ldb = sw[4];
+#pragma unroll 4
+    for (int64_t k = 0; k < K; ++k) {
+      T w[NW];
+      if (ROWS) {  // line = row m; j runs over the NW columns
+        const T av = TA ? a[k * lda + line] : a[line * lda + k];
+#pragma unroll
+        for (int j = 0; j < NW; ++j) w[j] = TB ? b[j * ldb + k] : b[k * ldb + j];
+#pragma unroll
+        for (int j = 0; j < NW; ++j) acc[j] += av * w[j];
+      } else {  // line = column n; j runs over the NW rows
+        const T bv = TB ? b[line * ldb + k] : b[k * ldb + line];
+#pragma unroll
+        for (int j = 0; j < NW; ++j) w[j] = TA ? a[k * lda + j] : a[j * lda + k];
+#pragma unroll
+        for (int j = 0; j < NW; ++j) acc[j] += w[j] * bv;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NW; ++j) out[c_off + (ROWS ? line * ldc + j : j * ldc + line)] = acc[j];
+}
+
 template <typename T, bool TA, bool TB, bool ROWS>
 __global__ __launch_bounds__(kThreads) void gemm_skinny_kernel(
     const T* __restrict__ A, const T* __restrict__ B, T* __restrict__ C, T* __restrict__ W,
@@ -471,34 +508,20 @@ __global__ __launch_bounds__(kThreads) void gemm_skinny_kernel(
   const int64_t* pw = probs + static_cast<int64_t>(p) * TN_PROB_WORDS;
   const int64_t M = pw[0], N = pw[1], c_off = pw[2], ldc = pw[3];
   const int64_t nl = ROWS ? M : N;   // lines: output rows (ROWS) or columns
-  const int nw = static_cast<int>(ROWS ? N : M);  // width of a line, <= kSkinnyMax
-  if (line >= nl || nw > kSkinnyMax) return;
-  T acc[kSkinnyMax];
-#pragma unroll
-  for (int j = 0; j < kSkinnyMax; ++j) acc[j] = T(0);
-  for (int64_t s = pw[4]; s < pw[5]; ++s) {
-    const int64_t* sw = segs + s * TN_SEG_WORDS;
-    const T* a = A + sw[0];
-    const T* b = B + sw[1];
-    const int64_t K = sw[2], lda = sw[3], ldb = sw[4];
-    for (int64_t k = 0; k < K; ++k) {
-      if (ROWS) {  // line = row m; j runs over the <= 8 columns
-        const T av = TA ? a[k * lda + line] : a[line * lda + k];
-#pragma unroll
-        for (int j = 0; j < kSkinnyMax; ++j)
-          if (j < nw) acc[j] += av * (TB ? b[j * ldb + k] : b[k * ldb + j]);
-      } else {  // line = column n; j runs over the <= 8 rows
-        const T bv = TB ? b[line * ldb + k] : b[k * ldb + line];
-#pragma unroll
-        for (int j = 0; j < kSkinnyMax; ++j)
-          if (j < nw) acc[j] += (TA ? a[k * lda + j] : a[j * lda + k]) * bv;
-      }
-    }
-  }
+  const int nw = static_cast<int>(ROWS ? N : M);  // width of a line, 1..kSkinnyMax
+  if (line >= nl) return;
   T* __restrict__ out = pw[6] ? W : C;
-#pragma unroll
-  for (int j = 0; j < kSkinnyMax; ++j)
-    if (j < nw) out[c_off + (ROWS ? line * ldc + j : j * ldc + line)] = acc[j];
+  switch (nw) {  // uniform over the workgroup: one problem per tile
+#define TN_SKINNY_NW(n) \
+  case n:               \
+    skinny_line<T, TA, TB, ROWS, n>(A, B, out, pw, segs, line, c_off, ldc); \
+    break;
+    TN_SKINNY_NW(1) TN_SKINNY_NW(2) TN_SKINNY_NW(3) TN_SKINNY_NW(4)
+    TN_SKINNY_NW(5) TN_SKINNY_NW(6) TN_SKINNY_NW(7) TN_SKINNY_NW(8)
+#undef TN_SKINNY_NW
+    default:
+      break;  // the host never plans a skinny problem outside 1..kSkinnyMax
+  }
 }
 
 template <typename T, bool ROWS>

@@ -34,7 +34,8 @@ __device__ __forceinline__ void linear_tile(const TS* __restrict__ src, TD* __re
                                             int64_t tile) {
   const int tid = threadIdx.x;
   const int64_t base = tile * (kThreads * PER);
-  int64_t so[PER], dof[PER];
+  // offsets within the region: below 2^31 (the host refuses larger regions)
+  int32_t so[PER], dof[PER];
   bool live[PER];
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
@@ -51,20 +52,35 @@ __device__ __forceinline__ void linear_tile(const TS* __restrict__ src, TD* __re
       d += i * dstr[k];
     }
     // what is left indexes dim 0 (e < nelem bounds it by shape[0])
-    so[r] = s + static_cast<int64_t>(rem) * sstr[0];
-    dof[r] = d + static_cast<int64_t>(rem) * dstr[0];
+    so[r] = static_cast<int32_t>(s + static_cast<int64_t>(rem) * sstr[0]);
+    dof[r] = static_cast<int32_t>(d + static_cast<int64_t>(rem) * dstr[0]);
   }
+  // the first term's loads land in the accumulators; a further term (multi-term SU(2)
+  // jobs, summed K slices) loads into CH more registers
+  constexpr int CH = PER;
   T acc[PER];
+  if (tb < te) {
+    const T c = static_cast<T>(tco[tb]);
+    const TS* __restrict__ s0 = src + tsrc[tb];
 #pragma unroll
-  for (int r = 0; r < PER; ++r) acc[r] = T(0);
-  for (int64_t t = tb; t < te; ++t) {
+    for (int r = 0; r < PER; ++r) acc[r] = static_cast<T>(s0[so[r]]);
+#pragma unroll
+    for (int r = 0; r < PER; ++r) acc[r] = T(0) + c * acc[r];
+  } else {
+#pragma unroll
+    for (int r = 0; r < PER; ++r) acc[r] = T(0);
+  }
+  for (int64_t t = tb + 1; t < te; ++t) {
     const T c = static_cast<T>(tco[t]);
     const TS* __restrict__ s0 = src + tsrc[t];
-    T v[PER];
 #pragma unroll
-    for (int r = 0; r < PER; ++r) v[r] = static_cast<T>(s0[so[r]]);
+    for (int r0 = 0; r0 < PER; r0 += CH) {
+      T v[CH];
 #pragma unroll
-    for (int r = 0; r < PER; ++r) acc[r] += c * v[r];
+      for (int q = 0; q < CH; ++q) v[q] = static_cast<T>(s0[so[r0 + q]]);
+#pragma unroll
+      for (int q = 0; q < CH; ++q) acc[r0 + q] += c * v[q];
+    }
   }
 #pragma unroll
   for (int r = 0; r < PER; ++r)

@@ -46,7 +46,8 @@ __all__ = [
     "require_device",
 ]
 
-_LIN_TILE = 2048
+_LIN_TILE = 2048        # linear tile of mode -1 (256 threads x 8 elements)
+_LIN_SMALL_TILE = 512   # linear tile of mode -2 (256 threads x 2 elements)
 _LDS_T = 64
 _SMALL_PROGRAM_TILES = 2048  # below this many workgroups, LDS jobs use 32 x 32 tiles
 _GEMM_PRECISION = {"value": os.environ.get("TENET_GEMM_PRECISION", "fp32")}
@@ -167,6 +168,9 @@ class RecoupleProgram:
             if doff < 0 or hi_dst >= dst_size:
                 raise ValueError("recouple region writes outside its destination buffer")
             span = sum((a - 1) * b for a, b in zip(shp, ss))
+            # the kernel addresses elements within a region with 32-bit offsets
+            if max(span, hi_dst - doff) >= 2**31:
+                raise ValueError("recouple region spans 2^31 elements or more")
             for so, _ in terms:
                 if so < 0 or so + span >= src_size:
                     raise ValueError("recouple term reads outside its source buffer")
@@ -183,7 +187,7 @@ class RecoupleProgram:
                 nt = (-(-shp[-1] // e)) * (-(-shp[mode] // e)) * (nelem // (shp[-1] * shp[mode]))
             elif mode < 0 and small:
                 jobs[j, 22] = -2
-                nt = -(-nelem // (_LIN_TILE // 4))
+                nt = -(-nelem // _LIN_SMALL_TILE)
             counts[j] = nt
         # tile table: (job, tile index within the job) for every job's tiles, in job order
         total = int(counts.sum())
