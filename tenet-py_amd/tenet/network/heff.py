@@ -152,7 +152,7 @@ class _Chain:
                 timed(f"recouple{k}", prog, cur, nxt)
                 cur = nxt
             elif cur.dtype != wt:
-                cur = cur.to(wt)
+                cur = ops.cast_data(cur, wt)
             bp = data if prepared else ex.prepare_b(data)
             ev = None if events is None else events.setdefault(k, [])
             rev = None if events is None else events.setdefault(f"reduce{k}", [])
@@ -168,7 +168,7 @@ class _Chain:
             out = torch.empty(self.s_out.size, dtype=x.dtype, device=x.device)
             timed("final", self.final, cur, out)
             return out
-        return cur if cur.dtype == x.dtype else cur.to(x.dtype)
+        return ops.cast_data(cur, x.dtype)
 
 
 def _dense(t: SymmetricTensor):

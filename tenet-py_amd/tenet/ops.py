@@ -371,17 +371,29 @@ def conj(t: SymmetricTensor) -> SymmetricTensor:
     return t
 
 
+def cast_data(x: torch.Tensor, dtype: torch.dtype) -> torch.Tensor:
+    """A flat FP64/FP32 device buffer converted by ``tn_cast`` (``x`` itself when it already
+    has ``dtype``)."""
+    if x.dtype == dtype:
+        return x
+    codes = {torch.float64: _native.TN_F64, torch.float32: _native.TN_F32}
+    if x.dtype not in codes or dtype not in codes:
+        raise TypeError(f"cast_data: {x.dtype} -> {dtype} is not an FP64/FP32 conversion")
+    if x.device.type != "cuda" or not x.is_contiguous():
+        raise RuntimeError("cast_data: needs a contiguous buffer on the HIP device")
+    out = torch.empty(x.numel(), dtype=dtype, device=x.device)
+    _native.check(
+        _native.lib().tn_cast(codes[x.dtype], x.data_ptr(), codes[dtype], out.data_ptr(), x.numel(),
+                              _native.stream_ptr())
+    )
+    return out
+
+
 def astype(t: SymmetricTensor, dtype: torch.dtype) -> SymmetricTensor:
     if t.dtype == dtype:
         return t
     require_device(t)
-    out = torch.empty(t.data.numel(), dtype=dtype, device=t.device)
-    dst = _native.TN_F64 if dtype == torch.float64 else _native.TN_F32
-    _native.check(
-        _native.lib().tn_cast(_dt(t), t.data.data_ptr(), dst, out.data_ptr(), t.data.numel(),
-                              _native.stream_ptr())
-    )
-    return SymmetricTensor(t.structure, out)
+    return SymmetricTensor(t.structure, cast_data(t.data.reshape(-1), dtype))
 
 
 def allclose(a: SymmetricTensor, b: SymmetricTensor, rtol: float = 1e-5, atol: float = 1e-8) -> bool:
