@@ -284,6 +284,9 @@ class MultiProcessServer:
                     results.append(replace(val, admitted=val.admitted - t0, finished=val.finished - t0))
                     if on_result is not None:
                         on_result(results[-1])
+            if not closed:  # an empty stream: the workers still need their stop message
+                for _ in procs:
+                    req_q.put(None)
             for p in procs:
                 p.join(timeout=60)
             return sorted(results, key=lambda r: r.rid)

@@ -98,3 +98,10 @@ def test_worker_failure_mid_stream_is_reported():
     srv = MultiProcessServer(workers=2, max_active=2, factory=partial(BrokenServer, 1, 1 << 40))
     with pytest.raises(RuntimeError, match="bond step failed"):
         srv.serve(_reqs(3), boot_timeout=120)
+
+
+def test_empty_stream_returns_at_once():
+    srv = MultiProcessServer(workers=2, max_active=2, factory=partial(FakeServer, 1, 1 << 40))
+    t0 = time.time()
+    assert srv.serve([], boot_timeout=120) == []
+    assert time.time() - t0 < 30  # the workers stop on their message, not on a join timeout
