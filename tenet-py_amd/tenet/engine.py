@@ -83,8 +83,16 @@ def _dtype_code(dtype: torch.dtype) -> int:
     raise TypeError(f"unsupported dtype {dtype}")
 
 
+_TORCH_DTYPES = {np.dtype(np.int64): torch.int64, np.dtype(np.int32): torch.int32,
+                 np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32}
+
+
 class _DeviceCache:
-    """Lazily uploaded copies of host tables, one per device."""
+    """Lazily uploaded copies of host tables, one per device.  All of a cache's tables
+    travel in ONE host-to-device copy (packed at 256-byte aligned offsets into one byte
+    buffer) and are handed out as typed views of it."""
+
+    _ALIGN = 256
 
     def __init__(self, **arrays) -> None:
         self.host = arrays
@@ -94,7 +102,21 @@ class _DeviceCache:
         key = (device.type, device.index)
         d = self._dev.get(key)
         if d is None:
-            d = {k: torch.from_numpy(np.ascontiguousarray(v)).to(device) for k, v in self.host.items()}
+            parts, spans, off = [], [], 0
+            for k, v in self.host.items():
+                a = np.ascontiguousarray(v)
+                if a.dtype not in _TORCH_DTYPES:
+                    raise TypeError(f"device table {k!r}: unsupported dtype {a.dtype}")
+                spans.append((k, off, a.size, a.dtype))
+                parts.append((off, a.view(np.uint8).reshape(-1)))
+                off += -(-a.nbytes // self._ALIGN) * self._ALIGN
+            raw = np.zeros(max(off, self._ALIGN), dtype=np.uint8)
+            for o, b in parts:
+                raw[o:o + b.size] = b
+            buf = torch.from_numpy(raw).to(device)
+            d = {}
+            for k, o, n, dt in spans:
+                d[k] = buf[o:o + n * dt.itemsize].view(_TORCH_DTYPES[dt])
             self._dev[key] = d
         return d
 
