@@ -148,69 +148,66 @@ class RecoupleProgram:
 
     def __init__(self, regions, src_size: int, dst_size: int) -> None:
         n = len(regions)
-        jobs = np.zeros((n, JOB_WORDS), dtype=np.int64)
-        tsrc, tco, tiles = [], [], []
+        rows = []  # job words, one list per region (the table is built in one step)
+        tsrc, tco, meta = [], [], []
         tb = 0
-        self.elements = 0
-        self.moved_terms = 0
-        for j, (doff, shape, dstr, sstr, terms) in enumerate(regions):
+        elements = moved = 0
+        pad = [0] * MAX_RANK
+        for doff, shape, dstr, sstr, terms in regions:
             shp, ds, ss = _normalize(shape, dstr, sstr)
-            if len(shp) > MAX_RANK:
-                raise ValueError(f"recouple region of rank {len(shp)} exceeds {MAX_RANK}")
             r = len(shp)
+            if r > MAX_RANK:
+                raise ValueError(f"recouple region of rank {r} exceeds {MAX_RANK}")
             nelem = math.prod(shp)
             mode = -1
             if ss[-1] != 1 and ds[-1] == 1 and shp[-1] >= 16:
-                cands = [k for k in range(r - 1) if ss[k] == 1 and shp[k] >= 16]
-                if cands:
-                    mode = cands[0]
-            jobs[j, 0] = doff
-            jobs[j, 1] = tb
-            jobs[j, 2] = tb + len(terms)
-            jobs[j, 3] = r
-            jobs[j, 4 : 4 + r] = shp
-            jobs[j, 10 : 10 + r] = ds
-            jobs[j, 16 : 16 + r] = ss
-            jobs[j, 22] = mode
-            jobs[j, 23] = nelem
+                for k in range(r - 1):
+                    if ss[k] == 1 and shp[k] >= 16:
+                        mode = k
+                        break
+            nterms = len(terms)
+            fill = pad[: MAX_RANK - r]
+            rows.append([doff, tb, tb + nterms, r, *shp, *fill, *ds, *fill, *ss, *fill, mode, nelem])
             for so, c in terms:
                 tsrc.append(so)
                 tco.append(float(c.real) if isinstance(c, complex) else float(c))
-            tb += len(terms)
-            if mode < 0:
-                nt = -(-nelem // _LIN_TILE)
-            else:
-                batch = nelem // (shp[-1] * shp[mode])
-                nt = (-(-shp[-1] // _LDS_T)) * (-(-shp[mode] // _LDS_T)) * batch
-            tiles.append((j, nt, mode, shp, nelem))
-            self.elements += nelem
-            self.moved_terms += nelem * len(terms)
+            tb += nterms
+            meta.append((mode, shp, nelem))
+            elements += nelem
+            moved += nelem * nterms
             # every access stays in bounds: checked here once, on the host
-            hi_dst = doff + sum((a - 1) * b for a, b in zip(shp, ds))
-            if doff < 0 or hi_dst >= dst_size:
+            dspan = sum((x - 1) * y for x, y in zip(shp, ds))
+            if doff < 0 or doff + dspan >= dst_size:
                 raise ValueError("recouple region writes outside its destination buffer")
-            span = sum((a - 1) * b for a, b in zip(shp, ss))
+            span = sum((x - 1) * y for x, y in zip(shp, ss))
             # the kernel addresses elements within a region with 32-bit offsets
-            if max(span, hi_dst - doff) >= 2**31:
+            if max(span, dspan) >= 2**31:
                 raise ValueError("recouple region spans 2^31 elements or more")
             for so, _ in terms:
                 if so < 0 or so + span >= src_size:
                     raise ValueError("recouple term reads outside its source buffer")
+        self.elements, self.moved_terms = elements, moved
         self.njobs = n
+        jobs = np.asarray(rows, dtype=np.int64).reshape(n, JOB_WORDS)
+
+        def large_tiles(mode, shp, nelem):
+            if mode < 0:
+                return -(-nelem // _LIN_TILE)
+            return (-(-shp[-1] // _LDS_T)) * (-(-shp[mode] // _LDS_T)) * (nelem // (shp[-1] * shp[mode]))
+
         # a program of few large regions would leave most of the device idle: give its LDS
         # jobs 32 x 32 tiles, four times the workgroups
         # (mode word J + 8); its linear jobs likewise get 512-element tiles (mode word -2)
-        small = sum(t[1] for t in tiles) < _SMALL_PROGRAM_TILES
-        counts = np.zeros(n, dtype=np.int64)
-        for j, nt, mode, shp, nelem in tiles:
-            if mode >= 0 and small:
-                jobs[j, 22] = mode + 8
-                e = _LDS_T // 2
-                nt = (-(-shp[-1] // e)) * (-(-shp[mode] // e)) * (nelem // (shp[-1] * shp[mode]))
-            elif mode < 0 and small:
-                jobs[j, 22] = -2
-                nt = -(-nelem // _LIN_SMALL_TILE)
-            counts[j] = nt
+        counts = np.array([large_tiles(*m) for m in meta], dtype=np.int64)
+        if int(counts.sum()) < _SMALL_PROGRAM_TILES:
+            e = _LDS_T // 2
+            for j, (mode, shp, nelem) in enumerate(meta):
+                if mode >= 0:
+                    jobs[j, 22] = mode + 8
+                    counts[j] = (-(-shp[-1] // e)) * (-(-shp[mode] // e)) * (nelem // (shp[-1] * shp[mode]))
+                else:
+                    jobs[j, 22] = -2
+                    counts[j] = -(-nelem // _LIN_SMALL_TILE)
         # tile table: (job, tile index within the job) for every job's tiles, in job order
         total = int(counts.sum())
         starts = np.cumsum(counts) - counts
@@ -265,21 +262,14 @@ def regions_for_plan(S: TensorStructure, D: TensorStructure, perm, terms):
             "coefficients dropped terms"
         )
     sl, dl = S.layout, D.layout
-    regions = []
-    for j in range(D.num_blocks):
-        lst = by_dst[j]
-        i0 = lst[0][0]
-        sstr = tuple(int(sl.strides[i0][p]) for p in perm)
-        regions.append(
-            (
-                int(dl.offsets[j]),
-                tuple(int(x) for x in dl.shapes[j]),
-                tuple(int(x) for x in dl.strides[j]),
-                sstr,
-                [(int(sl.offsets[i]), c) for i, c in lst],
-            )
-        )
-    return regions
+    s_off = sl.offsets.tolist()
+    s_str = np.asarray(sl.strides)[:, list(perm)].tolist() if len(perm) else [[] for _ in s_off]
+    d_off = dl.offsets.tolist()
+    d_shp = dl.shapes.tolist()
+    d_str = dl.strides.tolist()
+    return [(d_off[j], tuple(d_shp[j]), tuple(d_str[j]), tuple(s_str[by_dst[j][0][0]]),
+             [(s_off[i], c) for i, c in by_dst[j]])
+            for j in range(D.num_blocks)]
 
 
 @lru_cache(maxsize=4096)
