@@ -73,3 +73,21 @@ def test_f_move_unitarity_and_pentagon_beyond_fixture_range():
         if checked > 300:
             break
     assert checked > 50
+
+
+def test_six_j_against_sympy_exact_high_spins():
+    """Spins above the reference fixture's 2j <= 12 (the SU(2) workload's odd bonds reach
+    2j = 13 and their inner lines more): sympy's exact 6j symbols."""
+    from sympy import Rational
+    from sympy.physics.wigner import wigner_6j
+
+    rng = np.random.default_rng(7)
+    done = 0
+    while done < 25:
+        js = tuple(int(x) for x in rng.integers(12, 27, 6))
+        j1, j2, j3, j4, j5, j6 = js
+        if not (C._tri(j1, j2, j3) and C._tri(j1, j5, j6) and C._tri(j4, j2, j6) and C._tri(j4, j5, j3)):
+            continue
+        ref = float(wigner_6j(*(Rational(j, 2) for j in js)))
+        assert abs(C.six_j(*js) - ref) < 1e-13
+        done += 1
