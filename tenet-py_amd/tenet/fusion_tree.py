@@ -8,7 +8,7 @@ field order defines the canonical sort that the storage order of blocks rests on
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from functools import cache
 
 __all__ = ["FusionTree", "fusion_trees", "coupled_sectors"]
@@ -22,6 +22,15 @@ class FusionTree:
     inner: tuple
     multiplicities: tuple
     coupled: object
+    # trees key every block lookup: the hash (the one the generated __hash__ would compute
+    # from the four fields) is computed once
+    _h: int = field(init=False, repr=False, compare=False)
+
+    def __post_init__(self) -> None:
+        object.__setattr__(self, "_h", hash((self.uncoupled, self.inner, self.multiplicities, self.coupled)))
+
+    def __hash__(self) -> int:
+        return self._h
 
     @property
     def rank(self) -> int:

@@ -20,7 +20,7 @@ move does not re-enumerate its blocks (reference ``structure.py:311-338``).
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from dataclasses import replace as _replace
 from functools import cache
 from itertools import product
@@ -39,6 +39,13 @@ class FusionBlockKey:
 
     output_tree: FusionTree
     input_tree: FusionTree
+    _h: int = field(init=False, repr=False, compare=False)  # the generated hash, computed once
+
+    def __post_init__(self) -> None:
+        object.__setattr__(self, "_h", hash((self.output_tree, self.input_tree)))
+
+    def __hash__(self) -> int:
+        return self._h
 
     @property
     def coupled(self):
