@@ -25,9 +25,27 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+
+// A double moved across lanes by one DPP pattern (both 32-bit halves), VALU only.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const i32x2 p = __builtin_bit_cast(i32x2, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, p.x, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, p.y, CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, i32x2{lo, hi});
+}
+
 __device__ inline double half_sum(double v) {
-  // butterfly within each 32-lane half of the wave
-  for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+  // all-reduce within each 32-lane half of the wave: lane ^ 1 and lane ^ 2 (quad_perm),
+  // the other quad of 8 lanes (row_half_mirror), the other 8 of 16 (row_mirror) -- DPP
+  // moves, no LDS crossbar -- then the other row of 16 (one permute).  Every step adds
+  // two partial sums that both partners hold, so all 32 lanes end with the same bits.
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  v += __shfl_xor(v, 16, 32);
   return v;
 }
 
