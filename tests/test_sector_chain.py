@@ -95,3 +95,36 @@ def test_planted_mpo_coefficient_error_is_caught():
     ref = N.heff2(*(to_oracle(t) for t in (fl, w1, w2, fr, aa)))
     # far outside the 1e-12 agreement of the correct tables (the flipped slab is small)
     assert rel_err(got, ref.flat()) > 1e-9
+
+
+def _region_cover(regions, size):
+    cover = np.zeros(size, dtype=np.int64)
+    for doff, shape, dstr, _, _ in regions:
+        idx = np.full(1, doff, dtype=np.int64)
+        for n, d in zip(shape, dstr):
+            idx = (idx[:, None] + np.arange(n, dtype=np.int64)[None, :] * d).ravel()
+        np.add.at(cover, idx, 1)
+    return cover
+
+
+@pytest.mark.parametrize("left,right,n_sites,dim,sigma,seed", [
+    (5, 7, 12, 40, 1.5, 3), (2, 4, 12, 64, 3.0, 11), (9, 11, 20, 512, 4.0, 5)])
+def test_every_intermediate_element_is_written_once(left, right, n_sites, dim, sigma, seed):
+    """The stacked x, t3 and the output are allocated uninitialised: the entry gather, the
+    MPO pass and the exit scatter must each write every element exactly once."""
+    fl, w1, w2, fr, aa = _problem(left, right, n_sites, dim, sigma, seed)
+    ch = _chain(fl, w1, w2, fr, aa)
+    assert (_region_cover(ch.xs_regions, ch.xs_size) == 1).all()
+    jobs, ins, outs, coef = ch.mpo_host
+    t3 = np.zeros(ch.t3_size, dtype=np.int64)
+    for R, S, _, _, ob, oe, _, _ in jobs:
+        for off, rs in outs[ob:oe]:
+            np.add.at(t3, (off + np.arange(R)[:, None] * rs + np.arange(S)[None, :]).ravel(), 1)
+    assert (t3 == 1).all()
+    h = ch.exit._tables.host
+    from tenet.engine import JOB_WORDS
+    ex = []
+    for j in np.asarray(h["jobs"]).reshape(-1, JOB_WORDS):
+        r = int(j[3])
+        ex.append((int(j[0]), tuple(int(x) for x in j[4:4 + r]), tuple(int(x) for x in j[10:10 + r]), None, None))
+    assert (_region_cover(ex, aa.structure.size) == 1).all()
