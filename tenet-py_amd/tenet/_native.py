@@ -22,6 +22,7 @@ MAX_RANK = 6
 PROB_WORDS = 8
 SEG_WORDS = 6
 MPO_JOB_WORDS = 8
+MPO_TILE = 1024  # positions per tn_mpo_apply tile (256 threads x 4)
 SVD_PROB_WORDS = 10
 SVD_MAX_W = 32
 SVD_MAX_K = 4096

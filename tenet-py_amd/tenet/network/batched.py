@@ -23,6 +23,8 @@ import math
 import numpy as np
 import torch
 
+from tenet._native import MPO_TILE
+
 from tenet import _native, ops
 from tenet.engine import GemmProgram, _DeviceCache
 from tenet.network.heff import Heff2
@@ -144,7 +146,7 @@ class BatchedHeff2:
             cj, ci, co, cc = ch.mpo_host
             for R, S, ib, ie, ob, oe, cf, _ in cj:
                 jobs.append((R, S, ib + i0, ie + i0, ob + o0, oe + o0, cf + c0, 0))
-                nt = -(-(R * S) // 1024)
+                nt = -(-(R * S) // MPO_TILE)
                 tl.append(np.stack([np.full(nt, len(jobs) - 1, np.int64), np.arange(nt, dtype=np.int64)], 1))
             ins += [(off + int(t1_off[i]), rs) for off, rs in ci]
             outs += [(off + int(t3_off[i]), rs) for off, rs in co]
@@ -203,7 +205,7 @@ class BatchedHeff2:
             cj, ci, co, cc = ch.mpo_host
             for R, S, ib, ie, ob, oe, cf, _ in cj:
                 jobs.append((R, S, ib + i0, ie + i0, ob + o0, oe + o0, cf + c0, 0))
-                nt = -(-(R * S) // 1024)
+                nt = -(-(R * S) // MPO_TILE)
                 tl.append(np.stack([np.full(nt, len(jobs) - 1, np.int64), np.arange(nt, dtype=np.int64)], 1))
             ins += [(off + int(t1_off[i]), rs) for off, rs in ci]
             outs += [(off + int(t3_off[i]), rs) for off, rs in co]

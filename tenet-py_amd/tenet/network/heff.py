@@ -19,6 +19,8 @@ from collections import OrderedDict
 import numpy as np
 import torch
 
+from tenet._native import MPO_TILE
+
 from tenet import ops
 from tenet.engine import contraction_exec, inplace_map, recouple_program, require_device
 from tenet.plans import repartition_plan
@@ -310,7 +312,7 @@ class _FusedChain:
             ins.extend((i[0], i[1]) for i in ilist)
             outs.extend((o[0], o[1]) for o in olist)
             coef.extend(cm.reshape(-1).tolist())
-            nt = -(-(R * S) // 1024)
+            nt = -(-(R * S) // MPO_TILE)
             tiles.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
             for off, rs, _, _, _ in ilist:
                 if off + (R - 1) * rs + S > s1.size:
@@ -387,7 +389,7 @@ def _mpo_jobs(inputs, outputs, w12, src_size, dst_size):
         ins.extend((i[0], i[1]) for i in ilist)
         outs.extend((o[0], o[1]) for o in olist)
         coef.extend(cm.reshape(-1).tolist())
-        nt = -(-(R * S) // 1024)
+        nt = -(-(R * S) // MPO_TILE)
         tiles.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
         for off, rs, _, r_, s_ in ilist:
             if r_ != R or s_ != S or off < 0 or off + (R - 1) * rs + S > src_size:
