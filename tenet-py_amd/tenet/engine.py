@@ -729,19 +729,26 @@ class ContractionExec:
         b_lookup = {}
         for j, key in enumerate(Bp.block_order):
             b_lookup[(key.output_tree, key.input_tree)] = j
+        # row / column / inner extents of every block, one vectorised product each
+        sc = np.asarray(shapes_c, dtype=np.int64).reshape(len(shapes_c), Cp.ndim)
+        sa_ = np.asarray(shapes_a, dtype=np.int64).reshape(len(shapes_a), Ap.ndim)
+        Ms = np.prod(sc[:, :n_out_a], axis=1).tolist()
+        Ns = np.prod(sc[:, n_out_a:], axis=1).tolist()
+        Ka = np.prod(sa_[:, n_out_a:], axis=1).tolist()
+        a_offs = np.asarray(self.a.offsets).tolist()
+        b_offs = np.asarray(self.b.offsets).tolist()
+        c_list = np.asarray(c_offs).tolist()
         problems = []
         for i, key in enumerate(Cp.block_order):
-            shp = shapes_c[i]
-            M = int(np.prod(shp[:n_out_a], dtype=np.int64)) if n_out_a else 1
-            N = int(np.prod(shp[n_out_a:], dtype=np.int64)) if len(shp) > n_out_a else 1
+            M, N = Ms[i], Ns[i]
             segs = []
             for kt, ja in a_rows.get((key.coupled, key.output_tree), ()):
                 jb = b_lookup.get((kt, key.input_tree))
                 if jb is None:
                     continue
-                K = int(np.prod(shapes_a[ja][n_out_a:], dtype=np.int64))
-                segs.append((int(self.a.offsets[ja]), int(self.b.offsets[jb]), K, K, N))
-            problems.append((M, N, int(c_offs[i]), N, segs))
+                K = Ka[ja]
+                segs.append((a_offs[ja], b_offs[jb], K, K, N))
+            problems.append((M, N, c_list[i], N, segs))
         a_size = sa.size if self.a.program is None else Ap.size
         b_size = sb.size if self.b.program is None else Bp.size
         self.gemm = GemmProgram(problems, a_size, b_size, c_size)
