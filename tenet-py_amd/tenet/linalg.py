@@ -51,6 +51,7 @@ class _MatrixLayout:
             acc += r * k
         self.size = acc
         bl = s.layout
+        b_shapes, b_strides, b_offs = bl.shapes.tolist(), bl.strides.tolist(), bl.offsets.tolist()
         n_out = len(s.out_axes)
         order = (*s.out_axes, *s.in_axes)
         to_regions, from_regions = [], []
@@ -60,8 +61,8 @@ class _MatrixLayout:
             base = self.offsets[c]
             for (_, ro, dr), row in zip(rb, cells):
                 for (_, co, dc), i in zip(cb, row):
-                    shape = tuple(int(bl.shapes[i][a]) for a in order)
-                    blk = tuple(int(bl.strides[i][a]) for a in order)
+                    shape = tuple(b_shapes[i][a] for a in order)
+                    blk = tuple(b_strides[i][a] for a in order)
                     mat = [0] * len(order)
                     st = ld
                     for d in range(n_out - 1, -1, -1):
@@ -72,8 +73,8 @@ class _MatrixLayout:
                         mat[d] = st
                         st *= shape[d]
                     moff = base + ro * ld + co
-                    to_regions.append((moff, shape, tuple(mat), blk, [(int(bl.offsets[i]), 1.0)]))
-                    from_regions.append((int(bl.offsets[i]), shape, blk, tuple(mat), [(moff, 1.0)]))
+                    to_regions.append((moff, shape, tuple(mat), blk, [(b_offs[i], 1.0)]))
+                    from_regions.append((b_offs[i], shape, blk, tuple(mat), [(moff, 1.0)]))
         self.to_prog = RecoupleProgram(to_regions, s.size, self.size)
         self.from_prog = RecoupleProgram(from_regions, self.size, s.size)
 
