@@ -111,12 +111,12 @@ def from_matrices(structure: TensorStructure, mats: dict) -> SymmetricTensor:
     if set(mats) != set(ml.sectors):
         raise ValueError("from_matrices: sector set does not match the structure")
     ref = next(iter(mats.values()))
-    buf = torch.empty(ml.size, dtype=ref.dtype, device=ref.device)
     for c in ml.sectors:
-        m = mats[c]
-        if tuple(m.shape) != tuple(ml.shapes[c]):
-            raise ValueError(f"from_matrices: sector {c!r} has shape {tuple(m.shape)}, expected {ml.shapes[c]}")
-        buf[ml.offsets[c] : ml.offsets[c] + m.numel()].view(ml.shapes[c]).copy_(m)
+        if tuple(mats[c].shape) != tuple(ml.shapes[c]):
+            raise ValueError(f"from_matrices: sector {c!r} has shape {tuple(mats[c].shape)}, "
+                             f"expected {ml.shapes[c]}")
+    # the packed buffer is the sector matrices back to back in sector order: one batched copy
+    buf = torch.cat([mats[c].reshape(-1) for c in ml.sectors])
     out = torch.empty(structure.size, dtype=ref.dtype, device=ref.device)
     ml.from_prog.run(buf, out)
     return SymmetricTensor(structure, out)
@@ -394,9 +394,34 @@ def _factors(m: SymmetricTensor, parts: dict, keep: dict, scale: float = 1.0):
     ss = TensorStructure((Leg(bond, OUT), Leg(bond, IN)))
     vs = TensorStructure((Leg(bond, OUT), *m.domain))
     u = from_matrices(us, {c: parts[c][0][:, :k] for c, k in keep.items()})
-    s = from_matrices(ss, {c: torch.diag(parts[c][1][:k] * scale) for c, k in keep.items()})
+    s = _diagonal(ss, {c: parts[c][1][:k] for c, k in keep.items()}, scale)
     vh = from_matrices(vs, {c: parts[c][2][:k, :] for c, k in keep.items()})
     return u, s, vh
+
+
+def _diagonal(ss: TensorStructure, values: dict, scale: float = 1.0) -> SymmetricTensor:
+    """The diagonal singular-value tensor of structure ``ss`` (one k x k block per sector)
+    from per-sector value vectors: every value scattered onto its diagonal position of the
+    packed sector matrices in one launch, instead of one diagonal matrix per sector."""
+    ml = _matrix_layout(ss)
+    if set(values) != set(ml.sectors):
+        raise ValueError("diagonal factor: sector set does not match the structure")
+    ref = next(iter(values.values()))
+    vals = torch.cat([values[c] for c in ml.sectors]) * scale
+    cache = ml.__dict__.setdefault("_diag_index", {})
+    key = (ref.device.type, ref.device.index)
+    idx = cache.get(key)
+    if idx is None:
+        idx = torch.as_tensor(np.concatenate([ml.offsets[c] + np.arange(ml.shapes[c][0]) * (ml.shapes[c][0] + 1)
+                                              for c in ml.sectors]).astype(np.int64), device=ref.device)
+        cache[key] = idx
+    if vals.numel() != idx.numel():
+        raise ValueError("diagonal factor: value counts do not match the sector sizes")
+    buf = torch.zeros(ml.size, dtype=ref.dtype, device=ref.device)
+    buf[idx] = vals
+    out = torch.empty(ss.size, dtype=ref.dtype, device=ref.device)
+    ml.from_prog.run(buf, out)
+    return SymmetricTensor(ss, out)
 
 
 def svd(t: SymmetricTensor, axes=None, *, bond: GradedSpace | None = None):
