@@ -4,8 +4,8 @@ DMRG path, on ``oracle.u1`` tensors:
 * ``heff2``       -- ``_heff2_full`` (reference ``network/env.py:416-432``)
 * ``env_update``  -- ``Env.update_`` site-tensor branch (``env.py:593-633``)
 * ``lanczos``     -- ``network/dmrg.py:106-227`` (no re-orthogonalization, as there)
-* ``svd_truncated`` with ``_spectrum``/``_admissible``/``_decide`` (``ops/linalg.py:1079-1324,
-  1441-1612``)
+* ``svd_truncated`` with ``_spectrum``/``_admissible``/``keep_rule`` (``ops/linalg.py:1079-1324,
+  1441-1612``); the keep rule takes a ``qdim`` weight, so it also states the SU(2) rule
 * ``lq``/``canonize`` (``linalg.py:549-620``, ``network/mps.py:312-352``)
 * ``sweep``/``dmrg`` (``network/dmrg.py:321-843``, noiseless)
 * ``mpo_from_w``  -- ``MPO.from_w`` (``mps.py:2780-2845``) for U(1).
@@ -65,7 +65,11 @@ def _spectrum(values: dict):
     return e
 
 
-def _admissible(spec, cutoff, mode):
+def _admissible(spec, cutoff, mode, qdim=None):
+    """Length of the prefix of ``spec`` that the cutoff admits (reference
+    ``ops/linalg.py:1100-1129``): bare sigma for abs/rel, qdim-weighted powers of sigma for
+    the cumulative modes, whose relative forms compare against the same power."""
+    qdim = qdim or (lambda c: 1.0)
     if cutoff is None:
         return len(spec)
     if mode == "abs":
@@ -73,7 +77,7 @@ def _admissible(spec, cutoff, mode):
     if mode == "rel":
         return sum(1 for s, _, _ in spec if s > cutoff * spec[0][0])
     p = 2 if mode in ("sum2", "rsum2") else 1
-    w = [s**p for s, _, _ in spec]
+    w = [qdim(c) * s**p for s, c, _ in spec]
     thr = cutoff * sum(w) if mode.startswith("r") else cutoff
     kept, dropped = len(spec), 0.0
     for x in reversed(w):
@@ -84,18 +88,29 @@ def _admissible(spec, cutoff, mode):
     return kept
 
 
-def decide(spec, max_bond, cutoff, mode="rsum2"):
-    """U(1) keep rule (qdim = 1): returns ({sector: kept}, discarded weight)."""
-    adm = _admissible(spec, cutoff, mode)
-    keep, dense = {}, 0
-    kept = 0
-    for s, c, _ in spec[:adm]:
-        if max_bond is not None and dense + 1 > max_bond:
+def keep_rule(spec, max_bond, cutoff, mode="rsum2", qdim=None):
+    """The truncation keep rule (reference ``ops/linalg.py:1274-1324``): walk the admitted
+    prefix, charging ``qdim(c)`` of the dense budget per value, and stop at the first value
+    that would overflow ``max_bond``.  Returns ``(kept entries, dense dimension, discarded
+    weight sum qdim sigma^2)``."""
+    qdim = qdim or (lambda c: 1.0)
+    adm = _admissible(spec, cutoff, mode, qdim)
+    kept, dense = [], 0.0
+    for e in spec[:adm]:
+        if max_bond is not None and dense + qdim(e[1]) > max_bond:
             break
-        dense += 1
+        dense += qdim(e[1])
+        kept.append(e)
+    dw = sum(qdim(c) * s * s for s, c, _ in spec[len(kept):])
+    return kept, dense, dw
+
+
+def decide(spec, max_bond, cutoff, mode="rsum2", qdim=None):
+    """The keep rule as ``({sector: kept count}, discarded weight)``."""
+    kept, _, dw = keep_rule(spec, max_bond, cutoff, mode, qdim)
+    keep = {}
+    for _, c, _ in kept:
         keep[c] = keep.get(c, 0) + 1
-        kept += 1
-    dw = sum(s * s for s, _, _ in spec[kept:])
     return keep, dw
 
 

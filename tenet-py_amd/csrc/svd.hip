@@ -19,6 +19,10 @@
 
 namespace {
 
+// Singular values at or below this are treated as exact zeros (their W rows are completed
+// to an orthonormal set instead of being divided by sigma).
+constexpr double kNullSigma = 1e-290;
+
 __device__ inline double wave_sum(double v) {
   // butterfly: every lane ends with the same value (IEEE addition commutes exactly)
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -199,7 +203,7 @@ __global__ void jacobi_finish(const int64_t* __restrict__ probs, const double* _
   for (int i = 0; i < k; ++i) {
     const int rk = rank[i];
     const double si = sig[i];
-    const double inv = si > 0.0 ? 1.0 / si : 0.0;
+    const double inv = si > kNullSigma ? 1.0 / si : 0.0;
     const double* xr = x + xo + static_cast<int64_t>(i) * L;
     const double* qr = q + qo + static_cast<int64_t>(i) * k;
     if (!cols) {
@@ -210,6 +214,57 @@ __global__ void jacobi_finish(const int64_t* __restrict__ probs, const double* _
       // A^T (k x L) = Q^T diag(sigma) W: U[:, rk] = W row i, Vh[rk, :] = Q row i
       for (int j = threadIdx.x; j < L; j += blockDim.x) U[uo + static_cast<int64_t>(j) * k + rk] = xr[j] * inv;
       for (int r = threadIdx.x; r < k; r += blockDim.x) Vh[vo + static_cast<int64_t>(rk) * k + r] = qr[r];
+    }
+  }
+  // Null space.  A zero singular value leaves its W row zero; the factor on that side must
+  // still be an isometry, so those rows (ranked last) are completed by Gram-Schmidt of unit
+  // vectors e_0, e_1, ... against every row already set (two passes), taking the first
+  // candidate whose residual keeps more than half the mean residual (L - d) / L, which the
+  // best unit vector always reaches.  Row r of W, element j, lives at
+  // Vh[r, j] (rows <= cols) or U[j, r] (rows > cols).
+  __syncthreads();
+  int live = 0;
+  for (int i = 0; i < k; ++i) live += sig[i] > kNullSigma;
+  if (live == k) return;
+  const int64_t wrow = cols ? 1 : L, wcol = cols ? k : 1;
+  double* Wb = cols ? U + uo : Vh + vo;
+  double* coef = sig;  // sigma is not needed past this point
+  __shared__ double part[16];
+  __syncthreads();
+  for (int d = live; d < k; ++d) {
+    double* vd = Wb + d * wrow;
+    for (int e = 0; e < L; ++e) {
+      for (int j = threadIdx.x; j < L; j += blockDim.x) vd[j * wcol] = j == e ? 1.0 : 0.0;
+      __syncthreads();
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int p = threadIdx.x; p < d; p += blockDim.x) {
+          const double* wp = Wb + p * wrow;
+          double acc = 0.0;
+          for (int j = 0; j < L; ++j) acc = fma(wp[j * wcol], vd[j * wcol], acc);
+          coef[p] = acc;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < L; j += blockDim.x) {
+          double v = vd[j * wcol];
+          for (int p = 0; p < d; ++p) v = fma(-coef[p], Wb[p * wrow + j * wcol], v);
+          vd[j * wcol] = v;
+        }
+        __syncthreads();
+      }
+      double acc = 0.0;
+      for (int j = threadIdx.x; j < L; j += blockDim.x) acc = fma(vd[j * wcol], vd[j * wcol], acc);
+      acc = wave_sum(acc);
+      if (lane == 0) part[wave] = acc;
+      __syncthreads();
+      double nrm2 = 0.0;
+      for (int w2 = 0; w2 < nw; ++w2) nrm2 += part[w2];
+      __syncthreads();
+      if (nrm2 > 0.5 * static_cast<double>(L - d) / static_cast<double>(L)) {
+        const double inv = 1.0 / sqrt(nrm2);
+        for (int j = threadIdx.x; j < L; j += blockDim.x) vd[j * wcol] *= inv;
+        __syncthreads();
+        break;
+      }
     }
   }
 }

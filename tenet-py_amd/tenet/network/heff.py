@@ -297,29 +297,7 @@ class _FusedChain:
                         off = int(t3_off[k]) + ((ix * dP + iP) * dQ + iQ) * ds
                         dense = (o1[0][x] + ix, o1[1][P] + iP, o2[1][Q] + iQ)
                         out_by_as.setdefault((a, sg), []).append((off, mx * dP * dQ * ds, dense, da, ds))
-        jobs, ins, outs, coef, tiles = [], [], [], [], []
-        for (a, sg), olist in out_by_as.items():
-            ilist = by_as_in.get((a, sg), [])
-            if len(ilist) > 32:
-                raise ValueError("two-site MPO slab pair has more than 32 input channels")
-            R, S = olist[0][3], olist[0][4]
-            cm = np.zeros((len(olist), 32), dtype=np.float64)
-            for oi, o in enumerate(olist):
-                for ki, i in enumerate(ilist):
-                    cm[oi, ki] = w12[o[2] + i[2]]
-            j = len(jobs)
-            jobs.append((R, S, len(ins), len(ins) + len(ilist), len(outs), len(outs) + len(olist), len(coef), 0))
-            ins.extend((i[0], i[1]) for i in ilist)
-            outs.extend((o[0], o[1]) for o in olist)
-            coef.extend(cm.reshape(-1).tolist())
-            nt = -(-(R * S) // MPO_TILE)
-            tiles.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
-            for off, rs, _, _, _ in ilist:
-                if off + (R - 1) * rs + S > s1.size:
-                    raise ValueError("MPO pass reads outside t1")
-            for off, rs, _, _, _ in olist:
-                if off + (R - 1) * rs + S > self.t3_size:
-                    raise ValueError("MPO pass writes outside t3")
+        jobs, ins, outs, coef, tiles = _mpo_jobs(by_as_in, out_by_as, w12, s1.size, self.t3_size)
         self.mpo_host = (jobs, ins, outs, coef)
         self.t1_size = s1.size
         self.mpo_njobs = len(jobs)
@@ -380,6 +358,9 @@ def _mpo_jobs(inputs, outputs, w12, src_size, dst_size):
         if len(ilist) > 32:
             raise ValueError("two-site MPO slab pair has more than 32 input channels")
         R, S = olist[0][3], olist[0][4]
+        # the kernel decodes a position inside a slab pair as a 32-bit (row, column) pair
+        if S <= 0 or R <= 0 or R * S >= 2**32:
+            raise ValueError(f"two-site MPO slab pair of {R} x {S} positions is outside the kernel's range")
         cm = np.zeros((len(olist), 32), dtype=np.float64)
         for oi, o in enumerate(olist):
             for ki, i in enumerate(ilist):

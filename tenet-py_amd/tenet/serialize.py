@@ -2,8 +2,9 @@
 ``serialize.py:209-346``), and the MPS checkpoint directory built on it (reference
 ``network/mps.py:567-676``) that lets a served solve resume.
 
-The file holds a JSON header -- format version, the legs (provider kind, sectors with
-degeneracies, side, dual flag, name) and, for providers whose block values depend on a
+The file holds a JSON header in the reference's schema -- format version, the legs (side,
+dual flag, name, and the space as provider ``{kind, name}`` plus sector fields with
+degeneracies) and, for providers whose block values depend on a
 coefficient convention (SU(2), fZ2), a gauge fingerprint -- plus one member per block.
 Nothing derived is written: block order and shapes are recomputed from the legs, and the
 header's block count is checked against that recomputation.  A file written under a
@@ -43,6 +44,9 @@ GAUGES = {
     "SU2": "3j=condon-shortley;cg=condon-shortley,m-descending;f=tks-su2irrep;r=(-1)^(a+b-c);fs=(-1)^2j",
     "fZ2": "f=1;r=fermionic;fs=+1;twist=(-1)^p;perm=koszul",
 }
+# the reference writes "su2=racah;<racah build fingerprint>" (symmetry/su2.py:55)
+REFERENCE_SU2_GAUGE_PREFIX = "su2=racah;"
+VERSION = "0.1.0"  # written as the header's "tenet" field, as the reference does
 
 
 def _sector_args(a) -> list:
@@ -56,28 +60,48 @@ def _sector_args(a) -> list:
 
 
 def _encode_leg(leg: Leg, axis: int) -> dict:
+    """The reference's leg record (``serialize.py:165-178``): side, dual flag, name, and
+    the space as ``{provider: {kind, name}, sectors: [[sector fields], degeneracy]}``."""
     if not (leg.name is None or isinstance(leg.name, (str, int))):
         raise TypeError(f"public axis {axis}: leg name {leg.name!r} cannot be written to JSON; "
                         "only None, str and int names are supported")
     kind = leg.provider.name
     if kind not in _PROVIDERS:
         raise ValueError(f"cannot serialize provider {kind!r}; known kinds are {sorted(_PROVIDERS)}")
-    return {"provider": kind, "sectors": [[_sector_args(a), m] for a, m in leg.space.sectors],
-            "side": "out" if leg.side is OUT else "in", "dual": bool(leg.dual), "name": leg.name}
+    return {"side": "out" if leg.side is OUT else "in", "dual": bool(leg.dual), "name": leg.name,
+            "space": {"provider": {"kind": kind, "name": kind},
+                      "sectors": [[_sector_args(a), int(m)] for a, m in leg.space.sectors]}}
 
 
 def _decode_leg(d: dict) -> Leg:
-    kind = d["provider"]
+    """A leg record in the reference's schema (``serialize.py:181-188``)."""
+    prov = d["space"]["provider"]
+    kind = prov["kind"]
     if kind not in _PROVIDERS:
         raise KeyError(f"unknown provider kind {kind!r}; known kinds are {sorted(_PROVIDERS)}")
-    space = GradedSpace.new(_PROVIDERS[kind], {_SECTOR_OF[kind](args): int(m) for args, m in d["sectors"]})
+    if prov.get("name", kind) != kind:
+        raise ValueError(f"provider {kind!r} renamed to {prov['name']!r} is a different symmetry; "
+                         "this build only has the default-named providers")
+    space = GradedSpace.new(_PROVIDERS[kind],
+                            {_SECTOR_OF[kind](args): int(m) for args, m in d["space"]["sectors"]})
     return Leg(space, OUT if d["side"] == "out" else IN, bool(d["dual"]), d.get("name"))
+
+
+def _gauge_accepted(kind: str, gauge: str) -> bool:
+    """This build's own fingerprint, or -- for SU(2) -- the reference's racah-derived one:
+    the coefficient tables here agree with racah's on every row of the reference's F and R
+    fixtures (to 4.4e-16), so such files are numerically comparable."""
+    running = GAUGES.get(kind)
+    if running is None or gauge == running:
+        return True
+    return kind == "SU2" and gauge.startswith(REFERENCE_SU2_GAUGE_PREFIX)
 
 
 def save(t, path: str | os.PathLike, *, compress: bool = False) -> None:
     """Write ``t`` to ``path``: a JSON header plus one array per block (host float64)."""
     header = {
         "format": FORMAT_VERSION,
+        "tenet": VERSION,
         "legs": [_encode_leg(leg, i) for i, leg in enumerate(t.legs)],
         "gauges": {k: GAUGES[k] for k in {leg.provider.name for leg in t.legs} if k in GAUGES},
         "num_blocks": t.structure.num_blocks,
@@ -99,7 +123,7 @@ def load(path: str | os.PathLike, *, device=None, dtype=torch.float64):
                              f"format {FORMAT_VERSION}")
         for kind, gauge in header.get("gauges", {}).items():
             running = GAUGES.get(kind)
-            if running is not None and gauge != running:
+            if not _gauge_accepted(kind, gauge):
                 raise ValueError(f"{path}: {kind} coefficient gauge on disk is {gauge!r}, but this build "
                                  f"runs {running!r}; the block coefficients are not comparable")
         structure = TensorStructure(tuple(_decode_leg(d) for d in header["legs"]))

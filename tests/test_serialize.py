@@ -90,3 +90,45 @@ def test_mps_checkpoint_round_trip_and_refusals(tmp_path):
         (d / "003.npz").unlink()
         with pytest.raises(ValueError, match="missing"):
             load_mps(d, device="cuda")
+
+
+def test_reads_a_header_in_the_reference_schema(tmp_path):
+    """A file laid out as the reference writes it (``serialize.py:165-188,259-264``): legs
+    ``{side, dual, name, space: {provider: {kind, name}, sectors}}``, a ``tenet`` version
+    field and, for SU(2), the reference's racah gauge fingerprint."""
+    from tenet import SymmetricTensor
+    from tenet.serialize import load
+
+    for kind, gauges in (("u1", {}), ("su2", {"SU2": "su2=racah;sun-authority=racah-0.1.1"})):
+        t = SymmetricTensor.random(_legs(kind), seed=9, device="cpu")
+        prov = "U1" if kind == "u1" else "SU2"
+        legs = []
+        for l in t.legs:
+            secs = [[[a.charge if kind == "u1" else a.two_j], m] for a, m in l.space.sectors]
+            legs.append({"side": l.side.value, "dual": l.dual, "name": l.name,
+                         "space": {"provider": {"kind": prov, "name": prov}, "sectors": secs}})
+        header = {"format": 1, "tenet": "0.1.0", "legs": legs, "gauges": gauges,
+                  "num_blocks": t.structure.num_blocks}
+        p = tmp_path / f"{kind}.npz"
+        np.savez(p, header=np.array(json.dumps(header)),
+                 **{f"b{i}": b for i, b in enumerate(t.numpy_blocks())})
+        back = load(p, device="cpu")
+        assert back.structure == t.structure
+        assert torch.equal(back.data, t.data)
+
+
+def test_writes_the_reference_schema(tmp_path):
+    from tenet import SymmetricTensor
+    from tenet.serialize import save
+
+    t = SymmetricTensor.random(_legs("fz2"), seed=2, device="cpu")
+    save(t, tmp_path / "t.npz")
+    with np.load(tmp_path / "t.npz", allow_pickle=False) as z:
+        header = json.loads(str(z["header"].item()))
+    assert header["tenet"] and header["format"] == 1
+    leg = header["legs"][1]
+    assert set(leg) == {"side", "dual", "name", "space"}
+    assert leg["space"]["provider"] == {"kind": "fZ2", "name": "fZ2"}
+    assert leg["space"]["sectors"] == [[[0], 2], [[1], 3]]
+    assert (leg["side"], leg["dual"], leg["name"]) == ("out", True, "p")
+    assert header["gauges"] == {"fZ2": "f=1;r=fermionic;fs=+1;twist=(-1)^p;perm=koszul"}
