@@ -95,6 +95,24 @@ int tn_gemm_grouped(int precision, int dtype, int trans_a, int trans_b, const vo
                     const int64_t* segs, const int32_t* tiles, int64_t ntiles, int bm, int bn,
                     int grid, void* stream);
 
+/* No counterpart in the reference: tn_gemm_grouped for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * split-K partial products go to a static device array of the library, at most
+ * tn_static_scratch_bytes() bytes (64 MiB; larger workspaces are refused with an error),
+ * instead of a caller-provided W; when red_ntiles > 0 a tn_recouple launch with the given
+ * job/term/tile tables (source = the scratch, destination = C) then sums the slices into C.
+ * ws_elems is the workspace the problem tables address, in elements of dtype.  Every slice
+ * element is written before it is read, so nothing needs clearing, but all calls on one
+ * device share that device's array: calls that overlap in time on one GPU (two streams,
+ * two threads) race and must be serialized by the caller.  No hipMalloc/hipFree. */
+int tn_gemm_grouped_static(int precision, int dtype, int trans_a, int trans_b, const void* A,
+                           const void* B, void* C, const int64_t* problems, const int64_t* segs,
+                           const int32_t* tiles, int64_t ntiles, int bm, int bn, int grid,
+                           int64_t ws_elems, const int64_t* red_jobs, const int64_t* red_src,
+                           const double* red_coeff, const int64_t* red_tiles, int64_t red_ntiles,
+                           void* stream);
+int64_t tn_static_scratch_bytes(void);
+
 /* The same grouped product for skinny problems (the W contractions of the generic two-site
  * H_eff, network/env.py:430-431: K and one output dimension of a few MPO/physical
  * channels): rows != 0 -- every problem has N <= 8, one thread per output row; rows == 0

@@ -25,6 +25,8 @@
 // C[(l/16)*4 + j][l%16]; FP64 results -- C[4*j + l/16][l%16].
 #include "tn_common.hpp"
 
+#include <mutex>
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -583,4 +585,64 @@ extern "C" int tn_gemm_grouped(int precision, int dtype, int trans_a, int trans_
   if (precision == TN_PREC_BF16X3)
     return launch_prec<TN_PREC_BF16X3>(trans_a, trans_b, bm, bn, A, B, C, W, problems, segs, tiles, nt, grid, s);
   return tn::fail("tn_gemm_grouped: unknown precision");
+}
+
+// ---- the entry point that allocates nothing ------------------------------------------
+// The split-K partial products of tn_gemm_grouped_static go to this file-scope array
+// instead of a caller-provided workspace.  Each device the library runs on holds its own
+// copy until the process ends; host code reaches it only through the per-device address
+// returned by hipGetSymbolAddress (the array's name is a host-side stand-in there).
+namespace {
+
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return tn::fail("tn_gemm_grouped_static: no current device");
+  if (dev < 0 || dev >= kMaxDevices) return tn::fail("tn_gemm_grouped_static: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+      return tn::fail(std::string("tn_gemm_grouped_static: hipGetSymbolAddress(scratch) failed: ") +
+                      hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int64_t tn_static_scratch_bytes(void) { return static_cast<int64_t>(kScratchBytes); }
+
+extern "C" int tn_gemm_grouped_static(int precision, int dtype, int trans_a, int trans_b,
+                                      const void* A, const void* B, void* C,
+                                      const int64_t* problems, const int64_t* segs,
+                                      const int32_t* tiles, int64_t ntiles, int bm, int bn, int grid,
+                                      int64_t ws_elems, const int64_t* red_jobs,
+                                      const int64_t* red_src, const double* red_coeff,
+                                      const int64_t* red_tiles, int64_t red_ntiles, void* stream) {
+  if (ws_elems < 0) return tn::fail("tn_gemm_grouped_static: negative workspace size");
+  const int64_t item = dtype == TN_F64 ? 8 : 4;
+  if (dtype != TN_F64 && dtype != TN_F32) return tn::fail("tn_gemm_grouped_static: unknown dtype");
+  if (ws_elems > static_cast<int64_t>(kScratchBytes) / item)
+    return tn::fail("tn_gemm_grouped_static: the split-K workspace needs " + std::to_string(ws_elems * item) +
+                    " bytes, more than the library's static scratch of " + std::to_string(kScratchBytes) +
+                    " bytes; use tn_gemm_grouped with a caller-provided workspace");
+  if (red_ntiles > 0 && ws_elems == 0)
+    return tn::fail("tn_gemm_grouped_static: a reduction needs a workspace");
+  void* w = nullptr;
+  if (int rc = scratch_for_current_device(&w)) return rc;
+  if (int rc = tn_gemm_grouped(precision, dtype, trans_a, trans_b, A, B, C, w, problems, segs, tiles,
+                               ntiles, bm, bn, grid, stream))
+    return rc;
+  if (red_ntiles <= 0) return 0;
+  return tn_recouple(dtype, w, C, red_jobs, red_src, red_coeff, red_tiles, red_ntiles, stream);
 }

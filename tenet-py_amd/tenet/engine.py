@@ -652,6 +652,36 @@ class GemmProgram:
                 reduce_events.append((r0, r1, self.reduce.bytes_moved(C.element_size())))
 
 
+    def run_static(self, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, precision: str | None = None,
+                   tile=None) -> None:
+        """The same product through ``tn_gemm_grouped_static``: the split-K partials go to
+        the library's static device scratch and the slice sum follows in the same call, so
+        nothing is allocated (for callers that cannot allocate).  Calls that overlap on one
+        device share that scratch and must be serialized by the caller."""
+        if self.nproblems == 0:
+            return
+        if self.skinny is not None or self.gather:
+            raise ValueError("run_static covers the MFMA grouped launch written to C, not skinny "
+                             "or gather-mode programs")
+        if not (A.dtype == B.dtype == C.dtype):
+            raise TypeError("gemm: operand dtypes differ")
+        if A.numel() < self.a_size or B.numel() < self.b_size or C.numel() < self.c_size:
+            raise ValueError("gemm: buffers smaller than the compiled plan")
+        prec = precision or gemm_precision(C.dtype)
+        tile = tuple(tile) if tile else self.tile
+        t, ntiles = self._tables(tile, C.device)
+        if self.reduce is not None and self.reduce.ntiles:
+            r = self.reduce._tables.get(C.device)
+            red = (r["jobs"].data_ptr(), r["tsrc"].data_ptr(), r["tco"].data_ptr(), r["tiles"].data_ptr(),
+                   self.reduce.ntiles)
+        else:
+            red = (None, None, None, None, 0)
+        _native.check(_native.lib().tn_gemm_grouped_static(
+            _native.PRECISIONS[prec], _dtype_code(C.dtype), int(self.trans_a), int(self.trans_b),
+            A.data_ptr(), B.data_ptr(), C.data_ptr(), t["probs"].data_ptr(), t["segs"].data_ptr(),
+            t["tiles"].data_ptr(), ntiles, tile[0], tile[1], 0, self.ws_size, *red, _native.stream_ptr()))
+
+
 # ----------------------------------------------------------------------------- contraction
 
 

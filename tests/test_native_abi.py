@@ -11,7 +11,7 @@ def test_library_exports_every_declared_symbol():
     from tenet import _native
 
     header = (ROOT / "include" / "tenet_amd.h").read_text()
-    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(tn_\w+)\s*\(", header, re.M))
+    declared = set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(tn_\w+)\s*\(", header, re.M))
     assert declared >= {"tn_recouple", "tn_gemm_grouped", "tn_dot", "tn_axpby"}
     lib = _native.load_library()
     for name in sorted(declared):
