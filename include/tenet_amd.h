@@ -64,7 +64,8 @@ extern "C" {
  *   [6] 0 = write to C, 1 = write to the workspace W (split-K partial)  [7] 0
  * Segment table: TN_SEG_WORDS int64 per K segment:
  *   [0] a_off [1] b_off [2] K [3] lda [4] ldb [5] 0
- * Tile table: 4 int32 per output tile: (problem, tile row, tile col, 0). */
+ * Tile table: 4 int32 per output tile: (problem, tile row, tile col, the problem's first
+ * segment [4]). */
 #define TN_PROB_WORDS 8
 #define TN_SEG_WORDS 6
 

@@ -132,20 +132,46 @@ __device__ __forceinline__ void mpo_body(const T* __restrict__ in, T* __restrict
                                          const int64_t* __restrict__ is, const int64_t* __restrict__ os,
                                          const T* __restrict__ cf, int nout, int64_t base, int64_t n,
                                          uint32_t S32) {
-  for (int j = 0; j < kMpoPer; ++j) {
-    const int64_t e = base + j * kThreads + threadIdx.x;
-    if (e >= n) break;
-    const uint32_t r = static_cast<uint32_t>(e) / S32;
-    const int64_t s = static_cast<int64_t>(static_cast<uint32_t>(e) - r * S32);
-    T v[NIN > 0 ? NIN : 1];
+  // PER positions at a time: all PER x NIN loads are issued before the first multiply, so a
+  // thread keeps that many loads in flight (a position past the job's end reads position
+  // 0, in bounds, and stores nothing)
+  constexpr int PER = NIN <= 8 ? kMpoPer : (NIN <= 16 ? 2 : 1);
 #pragma unroll
-    for (int k = 0; k < NIN; ++k) v[k] = in[is[2 * k] + r * is[2 * k + 1] + s];
+  for (int j0 = 0; j0 < kMpoPer; j0 += PER) {
+    int64_t pos[PER];
+    bool live[PER];
+    uint32_t rr[PER];
+    int64_t ss[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int64_t e = base + (j0 + u) * kThreads + threadIdx.x;
+      live[u] = e < n;
+      pos[u] = live[u] ? e : 0;
+      rr[u] = static_cast<uint32_t>(pos[u]) / S32;
+      ss[u] = static_cast<int64_t>(static_cast<uint32_t>(pos[u]) - rr[u] * S32);
+    }
+    T v[PER][NIN > 0 ? NIN : 1];
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) {
+      const int64_t o = is[2 * k], st = is[2 * k + 1];
+#pragma unroll
+      for (int u = 0; u < PER; ++u) v[u][k] = in[o + rr[u] * st + ss[u]];
+    }
     for (int o = 0; o < nout; ++o) {
       const T* row = cf + o * kMpoMax;
-      T acc = T(0);
+      const int64_t ob = os[2 * o], ost = os[2 * o + 1];
+      T acc[PER];
 #pragma unroll
-      for (int k = 0; k < NIN; ++k) acc += row[k] * v[k];
-      out[os[2 * o] + r * os[2 * o + 1] + s] = acc;
+      for (int u = 0; u < PER; ++u) acc[u] = T(0);
+#pragma unroll
+      for (int k = 0; k < NIN; ++k) {
+        const T c = row[k];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) acc[u] += c * v[u][k];
+      }
+#pragma unroll
+      for (int u = 0; u < PER; ++u)
+        if (live[u]) out[ob + rr[u] * ost + ss[u]] = acc[u];
     }
   }
 }

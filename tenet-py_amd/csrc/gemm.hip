@@ -421,6 +421,376 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(
   }
 }
 
+// ---- one tile per workgroup, two K steps of global loads in flight ----------------------
+// The launch form of tn_gemm_grouped with grid = 0 (FP32 and BF16x3; FP64 and the
+// persistent grid > 0 form run gemm_kernel above).  Each workgroup owns one output tile
+// and walks its K steps (across the problem's segments) with TWO register sets of staged
+// operands: while the MFMAs consume K step s from LDS, the loads of steps s + 1 and s + 2
+// are both in flight, so a K step has two steps' worth of MFMA time to hide its memory
+// latency instead of one.  The register sets alternate with the two LDS buffers (the loop
+// is unrolled by two so both sets stay in registers); one barrier per K step.
+//
+// Operand staging.  A quad is 4 elements moved by one thread.  When k is contiguous in
+// memory the quad is 4 consecutive k of one row (m of A, n of B), fetched with one 16-byte
+// load and stored k-contiguous in LDS ([row][k]); each lane then reads its 4 k values of an
+// MFMA step with one 16-byte LDS read.  When the row index is contiguous instead (A
+// transposed, B not transposed), the FP32 path fetches 4 consecutive rows at one k with one
+// 16-byte load and keeps LDS row-major in k ([k][row], row stride BM/BN + 4 so a lane group
+// of 16 rows at two k lands on all 32 banks); each lane reads its 4 k values with four
+// 4-byte LDS reads.  Either way one vector load moves 16 bytes, so the load issue rate
+// does not bound the K loop.  (BF16x3 keeps four 4-byte loads for row-contiguous operands:
+// its MFMA reads 8 consecutive k per lane.)
+template <int PREC, bool TA, bool TB, int BM, int BN>
+struct PipeSmem {
+  using T = typename Traits<PREC>::T;
+  static constexpr bool AMN = PREC == TN_PREC_FP32 && TA;
+  static constexpr bool BMN = PREC == TN_PREC_FP32 && !TB;
+  static constexpr int BK = Traits<PREC>::BK;
+  static constexpr int LK = BK + Traits<PREC>::PAD;
+  // [buf][row][k] for k-contiguous staging, [buf][k][row] for row-contiguous staging
+  static constexpr int A0 = AMN ? BK : BM, A1 = AMN ? BM + 4 : LK;
+  static constexpr int B0 = BMN ? BK : BN, B1 = BMN ? BN + 4 : LK;
+  T a[2][A0][A1];
+  T b[2][B0][B1];
+};
+template <bool TA, bool TB, int BM, int BN>
+struct PipeSmem<TN_PREC_BF16X3, TA, TB, BM, BN> : Smem<TN_PREC_BF16X3, BM, BN> {
+  static constexpr bool AMN = false;
+  static constexpr bool BMN = false;
+};
+
+template <int PREC, bool TA, bool TB, int BM, int BN>
+struct Stage {
+  using T = typename Traits<PREC>::T;
+  using V4 = T __attribute__((ext_vector_type(4)));
+  using SM_ = PipeSmem<PREC, TA, TB, BM, BN>;
+  static constexpr bool AMN = SM_::AMN, BMN = SM_::BMN;
+  static constexpr int BK = Traits<PREC>::BK;
+  static constexpr int QK = BK / 4;  // k quads per row of a K step
+  static constexpr int QA = (BM * QK + kThreads - 1) / kThreads;
+  static constexpr int QB = (BN * QK + kThreads - 1) / kThreads;
+  static constexpr bool QA_EXACT = BM * QK % kThreads == 0;
+  static constexpr bool QB_EXACT = BN * QK % kThreads == 0;
+  struct Regs {
+    V4 a[QA], b[QB];
+  };
+  // per quad: byte offset in the segment, row (or first row) and k (or first k) in the
+  // tile, and how many of its rows lie inside the problem (0: none, 4: all)
+  uint32_t ao[QA], bo[QB];
+  int am[QA], bn[QB];
+  int akq[QA], bkq[QB];
+  int arows[QA], brows[QB];
+  uint32_t astr, bstr;
+
+  __device__ __forceinline__ void positions(int tid, int m0, int n0, int M, int N) {
+#pragma unroll
+    for (int r = 0; r < QA; ++r) {
+      const int q = r * kThreads + tid;
+      int m, kq;
+      if (AMN) { m = (q % (BM / 4)) * 4; kq = q / (BM / 4); }
+      else if (!TA) { m = q / QK; kq = (q % QK) * 4; }
+      else { m = q % BM; kq = (q / BM) * 4; }
+      const bool live = QA_EXACT || q < BM * QK;
+      if (!live) { m = 0; kq = 0; }
+      am[r] = live ? m : -1;
+      akq[r] = kq;
+      const int in = M - (m0 + m);
+      arows[r] = !live || in <= 0 ? 0 : (AMN ? (in < 4 ? in : 4) : 4);
+    }
+#pragma unroll
+    for (int r = 0; r < QB; ++r) {
+      const int q = r * kThreads + tid;
+      int n, kq;
+      if (BMN) { n = (q % (BN / 4)) * 4; kq = q / (BN / 4); }
+      else if (TB) { n = q / QK; kq = (q % QK) * 4; }
+      else { n = q % BN; kq = (q / BN) * 4; }
+      const bool live = QB_EXACT || q < BN * QK;
+      if (!live) { n = 0; kq = 0; }
+      bn[r] = live ? n : -1;
+      bkq[r] = kq;
+      const int in = N - (n0 + n);
+      brows[r] = !live || in <= 0 ? 0 : (BMN ? (in < 4 ? in : 4) : 4);
+    }
+  }
+  // byte offsets of every quad inside a segment (32-bit: the host refuses segments that
+  // span 2 GiB or more); recomputed when the fetch cursor enters a new segment
+  __device__ __forceinline__ void offsets(int m0, int n0, int64_t lda, int64_t ldb) {
+    astr = static_cast<uint32_t>(lda * sizeof(T));
+    bstr = static_cast<uint32_t>(ldb * sizeof(T));
+#pragma unroll
+    for (int r = 0; r < QA; ++r) {
+      const int m = am[r] < 0 ? 0 : am[r];
+      const int64_t e = TA ? (int64_t)akq[r] * lda + (m0 + m) : (int64_t)(m0 + m) * lda + akq[r];
+      ao[r] = static_cast<uint32_t>(e * sizeof(T));
+    }
+#pragma unroll
+    for (int r = 0; r < QB; ++r) {
+      const int n = bn[r] < 0 ? 0 : bn[r];
+      const int64_t e = TB ? (int64_t)(n0 + n) * ldb + bkq[r] : (int64_t)bkq[r] * ldb + (n0 + n);
+      bo[r] = static_cast<uint32_t>(e * sizeof(T));
+    }
+  }
+  static __device__ __forceinline__ T load1(__amdgpu_buffer_rsrc_t d, uint32_t off) {
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(d, off, 0, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(d, off, 0, 0));
+  }
+  // 4 consecutive elements in memory (4 k of one row, or 4 rows at one k); `valid` of them
+  // lie inside the problem.  Elements outside read through an offset beyond the
+  // descriptor's range, which the hardware returns as zero.
+  static __device__ __forceinline__ V4 contig(__amdgpu_buffer_rsrc_t d, uint32_t off, int valid) {
+    V4 v;
+    if (valid >= 4) {
+      if constexpr (sizeof(T) == 8) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2 lo = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(d, off, 0, 0));
+        const d2 hi = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(d, off + 16, 0, 0));
+        v = V4{lo[0], lo[1], hi[0], hi[1]};
+      } else {
+        v = __builtin_bit_cast(V4, __builtin_amdgcn_raw_buffer_load_b128(d, off, 0, 0));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = load1(d, j < valid ? off + j * (uint32_t)sizeof(T) : 0x80000000u);
+    }
+    return v;
+  }
+  // 4 k values `str` bytes apart (lanes run along m / n, so each load is coalesced)
+  static __device__ __forceinline__ V4 strided(__amdgpu_buffer_rsrc_t d, uint32_t off, uint32_t str, bool ok,
+                                               int kq, int kleft) {
+    V4 v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = load1(d, (ok && kq + j < kleft) ? off + j * str : 0x80000000u);
+    return v;
+  }
+  __device__ __forceinline__ void fetch(Regs& rg, const T* Ap, const T* Bp, int kleft) const {
+    const __amdgpu_buffer_rsrc_t da = __builtin_amdgcn_make_buffer_rsrc((void*)Ap, (short)0, 0x7fffffff, 0x00020000);
+    const __amdgpu_buffer_rsrc_t db = __builtin_amdgcn_make_buffer_rsrc((void*)Bp, (short)0, 0x7fffffff, 0x00020000);
+#pragma unroll
+    for (int r = 0; r < QA; ++r) {
+      if (AMN) {
+        rg.a[r] = contig(da, ao[r], akq[r] < kleft ? arows[r] : 0);
+      } else if (TA) {
+        rg.a[r] = strided(da, ao[r], astr, arows[r] > 0, akq[r], kleft);
+      } else {
+        const int kv = kleft - akq[r];
+        rg.a[r] = contig(da, ao[r], arows[r] > 0 ? (kv < 4 ? (kv < 0 ? 0 : kv) : 4) : 0);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < QB; ++r) {
+      if (BMN) {
+        rg.b[r] = contig(db, bo[r], bkq[r] < kleft ? brows[r] : 0);
+      } else if (!TB) {
+        rg.b[r] = strided(db, bo[r], bstr, brows[r] > 0, bkq[r], kleft);
+      } else {
+        const int kv = kleft - bkq[r];
+        rg.b[r] = contig(db, bo[r], brows[r] > 0 ? (kv < 4 ? (kv < 0 ? 0 : kv) : 4) : 0);
+      }
+    }
+  }
+  __device__ __forceinline__ void stash(const Regs& rg, SM_& sm, int buf) const {
+#pragma unroll
+    for (int r = 0; r < QA; ++r) {
+      if (!QA_EXACT && am[r] < 0) continue;
+      if constexpr (PREC == TN_PREC_BF16X3) {
+        typedef unsigned short u4 __attribute__((ext_vector_type(4)));
+        u4 h, l;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          h[j] = bf16_rne(rg.a[r][j]);
+          l[j] = bf16_rne(rg.a[r][j] - bf16_to_f32(h[j]));
+        }
+        *reinterpret_cast<u4*>(&sm.ah[buf][am[r]][akq[r]]) = h;
+        *reinterpret_cast<u4*>(&sm.al[buf][am[r]][akq[r]]) = l;
+      } else if constexpr (AMN) {
+        *static_cast<V4*>(__builtin_assume_aligned(&sm.a[buf][akq[r]][am[r]], 16)) = rg.a[r];
+      } else {
+        *static_cast<V4*>(__builtin_assume_aligned(&sm.a[buf][am[r]][akq[r]], 16)) = rg.a[r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < QB; ++r) {
+      if (!QB_EXACT && bn[r] < 0) continue;
+      if constexpr (PREC == TN_PREC_BF16X3) {
+        typedef unsigned short u4 __attribute__((ext_vector_type(4)));
+        u4 h, l;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          h[j] = bf16_rne(rg.b[r][j]);
+          l[j] = bf16_rne(rg.b[r][j] - bf16_to_f32(h[j]));
+        }
+        *reinterpret_cast<u4*>(&sm.bh[buf][bn[r]][bkq[r]]) = h;
+        *reinterpret_cast<u4*>(&sm.bl[buf][bn[r]][bkq[r]]) = l;
+      } else if constexpr (BMN) {
+        *static_cast<V4*>(__builtin_assume_aligned(&sm.b[buf][bkq[r]][bn[r]], 16)) = rg.b[r];
+      } else {
+        *static_cast<V4*>(__builtin_assume_aligned(&sm.b[buf][bn[r]][bkq[r]], 16)) = rg.b[r];
+      }
+    }
+  }
+};
+
+// The MFMAs of one K step on LDS buffer `buf` for a wave owning rows wm0.. and columns wn0..
+template <int PREC, bool TA, bool TB, int BM, int BN, int SM, int SN>
+__device__ __forceinline__ void mfma_step(const PipeSmem<PREC, TA, TB, BM, BN>& sm, int buf, int wm0, int wn0,
+                                          int g, int r16, f32x4 (&acc)[SM][SN]) {
+  using SMT = PipeSmem<PREC, TA, TB, BM, BN>;
+  if constexpr (PREC == TN_PREC_BF16X3) {
+    const int kq = g * 8;
+    bf16x8 ah[SM], al[SM];
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      const int row = wm0 + i * 16 + r16;
+      ah[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.ah[buf][row][kq]));
+      al[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.al[buf][row][kq]));
+    }
+#pragma unroll
+    for (int j = 0; j < SN; ++j) {
+      const int col = wn0 + j * 16 + r16;
+      const bf16x8 bh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.bh[buf][col][kq]));
+      const bf16x8 bl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sm.bl[buf][col][kq]));
+#pragma unroll
+      for (int i = 0; i < SM; ++i) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+      }
+    }
+  } else {
+    constexpr int BK = Traits<PREC>::BK;
+#pragma unroll
+    for (int h = 0; h < BK / 16; ++h) {
+      const int kq = h * 16 + g * 4;
+      f32x4 a4[SM], b4[SN];
+#pragma unroll
+      for (int i = 0; i < SM; ++i) {
+        const int row = wm0 + i * 16 + r16;
+        if constexpr (SMT::AMN) {
+#pragma unroll
+          for (int tt = 0; tt < 4; ++tt) a4[i][tt] = sm.a[buf][kq + tt][row];
+        } else {
+          a4[i] = *reinterpret_cast<const f32x4*>(&sm.a[buf][row][kq]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < SN; ++j) {
+        const int col = wn0 + j * 16 + r16;
+        if constexpr (SMT::BMN) {
+#pragma unroll
+          for (int tt = 0; tt < 4; ++tt) b4[j][tt] = sm.b[buf][kq + tt][col];
+        } else {
+          b4[j] = *reinterpret_cast<const f32x4*>(&sm.b[buf][col][kq]);
+        }
+      }
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+        for (int i = 0; i < SM; ++i)
+#pragma unroll
+          for (int j = 0; j < SN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[i][tt], b4[j][tt], acc[i][j], 0, 0, 0);
+    }
+  }
+}
+
+template <int PREC, bool TA, bool TB, int BM, int BN>
+__global__ __launch_bounds__(kThreads, 4) void gemm_pipe_kernel(
+    const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ C, float* __restrict__ W,
+    const int64_t* __restrict__ probs, const int64_t* __restrict__ segs,
+    const int32_t* __restrict__ tiles, int ntiles) {
+  static_assert(PREC != TN_PREC_FP64, "FP64 runs gemm_kernel");
+  using St = Stage<PREC, TA, TB, BM, BN>;
+  constexpr int BK = Traits<PREC>::BK;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int SM = WM / 16, SN = WN / 16;
+  __shared__ __attribute__((aligned(16))) PipeSmem<PREC, TA, TB, BM, BN> sm;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave >> 1) * WM;
+  const int wn0 = (wave & 1) * WN;
+  const int g = lane >> 4;
+  const int r16 = lane & 15;
+
+  // workgroup b runs on XCD b % 8: hand XCD x the table slice [x G/8, (x+1) G/8)
+  const int G = gridDim.x;
+  int t = blockIdx.x;
+  if ((G & 7) == 0) t = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+  if (t >= ntiles) return;
+  // tile word 3 holds the problem's first segment, so the problem record and that segment
+  // are fetched side by side rather than one after the other
+  const int p = tiles[4 * t];
+  const int64_t* pw = probs + static_cast<int64_t>(p) * TN_PROB_WORDS;
+  const int m0 = tiles[4 * t + 1] * BM, n0 = tiles[4 * t + 2] * BN;
+  int64_t fs = tiles[4 * t + 3];
+  const int M = static_cast<int>(pw[0]), N = static_cast<int>(pw[1]);
+  const int64_t c_off = pw[2], ldc = pw[3];
+  const int64_t se = pw[5];
+  float* __restrict__ out = pw[6] ? W : C;
+
+  f32x4 acc[SM][SN];
+#pragma unroll
+  for (int i = 0; i < SM; ++i)
+#pragma unroll
+    for (int j = 0; j < SN; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+
+  St st;
+  st.positions(tid, m0, n0, M, N);
+  // fetch cursor: segment fs, k offset fk inside it (runs two K steps ahead of the MFMAs)
+  int fk = 0;
+  Seg fsg{};
+  if (fs < se) {
+    fsg = load_seg(segs, fs);
+    st.offsets(m0, n0, fsg.lda, fsg.ldb);
+  }
+  auto issue = [&](typename St::Regs& rg) -> bool {
+    // skip exhausted (or empty) segments; false once the problem's K range is done
+    while (fs < se && fk >= fsg.K) {
+      ++fs;
+      fk = 0;
+      if (fs < se) {
+        fsg = load_seg(segs, fs);
+        st.offsets(m0, n0, fsg.lda, fsg.ldb);
+      }
+    }
+    if (fs >= se) return false;
+    const float* Ap = A + fsg.a_off + (TA ? (int64_t)fk * fsg.lda : fk);
+    const float* Bp = B + fsg.b_off + (TB ? fk : (int64_t)fk * fsg.ldb);
+    st.fetch(rg, Ap, Bp, fsg.K - fk);
+    fk += BK;
+    return true;
+  };
+  typename St::Regs r0, r1;
+  bool h0 = issue(r0);
+  bool h1 = h0 && issue(r1);
+  while (h0) {
+    st.stash(r0, sm, 0);
+    __syncthreads();
+    h0 = issue(r0);
+    mfma_step<PREC, TA, TB, BM, BN, SM, SN>(sm, 0, wm0, wn0, g, r16, acc);
+    if (!h1) break;
+    st.stash(r1, sm, 1);
+    __syncthreads();
+    h1 = issue(r1);
+    mfma_step<PREC, TA, TB, BM, BN, SM, SN>(sm, 1, wm0, wn0, g, r16, acc);
+  }
+  // ---- epilogue: straight from the accumulators ----
+#pragma unroll
+  for (int i = 0; i < SM; ++i)
+#pragma unroll
+    for (int j = 0; j < SN; ++j) {
+      const int col = n0 + wn0 + j * 16 + r16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm0 + i * 16 + g * 4 + r;
+        if (row < M && col < N) out[c_off + row * ldc + col] = acc[i][j][r];
+      }
+    }
+}
+
 template <int PREC, bool TA, bool TB>
 int launch_tile(int bm, int bn, const void* A, const void* B, void* C, void* W,
                 const int64_t* probs, const int64_t* segs, const int32_t* tiles, int ntiles,
@@ -433,8 +803,18 @@ int launch_tile(int bm, int bn, const void* A, const void* B, void* C, void* W,
   const int g = grid > 0 ? grid : ntiles;
 #define TN_GEMM_CASE(BM_, BN_)                                                              \
   if (bm == BM_ && bn == BN_) {                                                             \
-    gemm_kernel<PREC, TA, TB, BM_, BN_><<<g, kThreads, 0, s>>>(a, b, c, w, probs, segs,     \
-                                                               tiles, ntiles);             \
+    if constexpr (PREC == TN_PREC_FP64) {                                                   \
+      gemm_kernel<PREC, TA, TB, BM_, BN_><<<g, kThreads, 0, s>>>(a, b, c, w, probs, segs,   \
+                                                                 tiles, ntiles);           \
+    } else {                                                                                \
+      if (grid > 0)                                                                         \
+        gemm_kernel<PREC, TA, TB, BM_, BN_><<<g, kThreads, 0, s>>>(a, b, c, w, probs, segs, \
+                                                                   tiles, ntiles);         \
+      else                                                                                  \
+        gemm_pipe_kernel<PREC, TA, TB, BM_, BN_><<<g, kThreads, 0, s>>>(a, b, c, w, probs,  \
+                                                                        segs, tiles,        \
+                                                                        ntiles);            \
+    }                                                                                       \
     return tn::check_launch("tn_gemm_grouped");                                            \
   }
   TN_GEMM_CASE(32, 32)

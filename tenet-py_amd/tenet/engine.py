@@ -362,6 +362,27 @@ def choose_tile(Ms, Ns, Ks) -> tuple:
     return (64, 64)
 
 
+def _band_layout(tiles: np.ndarray, nprob: int, nxcd: int = 8) -> np.ndarray:
+    """Tile table for one workgroup per tile: the tiles in (problem, tile row, tile col)
+    order cut into ``nxcd`` contiguous runs of equal work (K plus a per-tile fixed cost),
+    XCD x taking run x, each run ordered longest K first (ties in row-major order).  A
+    problem's tiles -- which share its operand panels -- land on one or two XCDs, so each
+    panel is fetched from HBM by at most two L2s; slices padded with empty-problem tiles."""
+    order = np.lexsort((tiles[:, 2], tiles[:, 1], tiles[:, 0]))
+    t = tiles[order]
+    work = t[:, 3].astype(np.float64) + 64.0
+    cum = np.cumsum(work)
+    bounds = np.searchsorted(cum, cum[-1] * np.arange(1, nxcd) / nxcd, side="right")
+    parts = np.split(np.arange(len(t)), bounds)
+    L = max(len(pt) for pt in parts)
+    out = np.zeros((nxcd * L, 4), dtype=tiles.dtype)
+    out[:, 0] = nprob
+    for x, pt in enumerate(parts):
+        pt = pt[np.argsort(-t[pt, 3], kind="stable")]
+        out[x * L: x * L + len(pt)] = t[pt]
+    return out
+
+
 def _xcd_layout(tiles: np.ndarray, nprob: int, nxcd: int = 8, grid: int = 0) -> np.ndarray:
     """Tile table for a launch of one workgroup per tile on a device whose workgroup b
     runs on XCD ``b % nxcd`` and whose GEMM kernel hands XCD x the table slice
@@ -563,12 +584,16 @@ class GemmProgram:
                 tiles = _xcd_layout(tiles, self.nproblems, grid=grid)
             elif self.XCD_LAYOUT == "lpt" and len(tiles) >= 64:
                 tiles = _xcd_layout(tiles, self.nproblems)
+            elif self.XCD_LAYOUT == "band" and len(tiles) >= 64:
+                tiles = _band_layout(tiles, self.nproblems)
             else:
                 # longest K first; ties keep problem order
                 tiles = tiles[np.argsort(-tiles[:, 3], kind="stable")]
                 if self.XCD_LAYOUT == "deal" and len(tiles) % 8 == 0:
                     tiles = tiles[np.arange(len(tiles)).reshape(-1, 8).T.reshape(-1)]
-            tiles[:, 3] = 0
+            # word 3: the problem's first segment (the kernel fetches it beside the problem)
+            seg0 = np.append(self._probs[:, 4], 0)
+            tiles[:, 3] = seg0[tiles[:, 0]]
             tiles = tiles.astype(np.int32)
             # one all-zero problem row past the real ones: the padding tiles of the XCD
             # layout point at it (M = N = 0, no segments: the workgroup exits at once)
