@@ -30,7 +30,7 @@ import torch
 from tenet import _native
 from tenet._native import JOB_WORDS, MAX_RANK, PROB_WORDS, SEG_WORDS
 from tenet.plans import (
-    _compose_terms,
+    compose_maps,
     contraction_plan,
     permutation_plan,
     repartition_plan,
@@ -714,7 +714,7 @@ def compose_plans(first, second):
     """(perm, terms) of applying plan ``first`` then plan ``second``."""
     p1, t1 = first
     p2, t2 = second
-    return tuple(p1[i] for i in p2), _compose_terms(t1, t2)
+    return tuple(p1[i] for i in p2), compose_maps(t1, t2)
 
 
 @dataclass

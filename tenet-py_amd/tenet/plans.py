@@ -1,13 +1,25 @@
-"""Host-side categorical plans: which source block lands in which target block, with what
-coefficient, under a permutation, a bend, a repartition or a contraction.
+"""Host-side categorical plans: where every source block goes, with which coefficient, when a
+tensor's legs are permuted, bent across the map, repartitioned or contracted.
 
-All plans are array-free and cached; the bodies that enumerate terms are cached on the
-degeneracy-free *pattern* of a structure, so moving bond degeneracies never re-derives
-coefficients (reference ``ops/permutation.py:137-197``, ``ops/repartition.py:117-385``).
+A *block map* is a tuple of ``(src_block, dst_block, coeff)`` triples -- a sparse linear
+map between the block index spaces of two structures.  Abelian symmetries give unit-weight
+bijections; SU(2) expands a source into several targets and sums several sources into
+one; fZ2 weights are +-1.  Plans are pure functions of frozen structures and are cached;
+the tree algebra underneath is evaluated on the degeneracy-free *pattern* of a structure
+(every degeneracy set to 1), so a bond whose degeneracies move reuses every coefficient
+(the caching scheme of reference ``ops/permutation.py:137-197`` and
+``ops/repartition.py:117-385``; the plans themselves follow the reference's semantics:
+``permutation_plan`` = ``permutation.py:160-197``, ``bend_plan`` = ``repartition.py:
+130-167``, ``repartition_plan`` = ``repartition.py:342-385``, ``contraction_plan`` =
+``contraction.py:248-306``, ``map_layout`` = ``map_view.py:155-235``).
 
-A term list is ``((src_block, dst_block, coeff), ...)``.  For U(1)/Trivial every term has
-coefficient exactly 1 and the map is one-to-one; SU(2) expands a source into several
-targets and several sources sum into one target.
+Design: each elementary move (a permutation of the public axes, or a bend of the last
+public axis) is a block map built from the provider's tree operations over every block
+key; a repartition is a *move list* derived from the legs' sides alone (every leg that
+changes side is carried to the last public position and bent, in ascending axis order,
+then one permutation puts the axes in the requested order) folded into a single map.
+Composition of maps is a join on the intermediate block index done with array
+arithmetic; unit-weight bijections compose by index lookup.
 """
 
 from __future__ import annotations
@@ -16,7 +28,9 @@ import operator
 from dataclasses import dataclass, replace
 from functools import cache
 
-from tenet.space import IN, OUT, Leg, Side
+import numpy as np
+
+from tenet.space import IN, OUT, Leg
 from tenet.structure import FusionBlockKey, TensorStructure, pattern_of
 from tenet.symmetry import CapabilityError
 
@@ -30,11 +44,78 @@ __all__ = [
     "repartition_plan",
     "contraction_plan",
     "map_layout",
+    "compose_maps",
     "validated_axes",
 ]
 
 
-# ----------------------------------------------------------------------------- transpose
+# ----------------------------------------------------------------------------- block maps
+
+
+def identity_map(n: int) -> tuple:
+    return tuple((i, i, 1.0) for i in range(n))
+
+
+def _unit_bijection(terms) -> bool:
+    """Every block goes to exactly one block with weight 1 (and no two to the same)."""
+    if any(c != 1 for _, _, c in terms):
+        return False
+    srcs = {s for s, _, _ in terms}
+    dsts = {d for _, d, _ in terms}
+    return len(srcs) == len(dsts) == len(terms)
+
+
+def compose_maps(first, second) -> tuple:
+    """The block map of applying ``first`` and then ``second``: every path
+    ``src -> mid -> dst`` contributes ``c_first * c_second``, contributions to one
+    ``(src, dst)`` pair summed in the order the paths are met (first's terms in order,
+    each through second's terms in order).  A pair whose paths cancel keeps a zero
+    weight, so every target block stays present in the map."""
+    if not first or not second:
+        return ()
+    if _unit_bijection(first) and _unit_bijection(second):
+        to = {m: d for m, d, _ in second}
+        return tuple((s, to[m], 1.0) for s, m, _ in first if m in to)
+    f = np.asarray(first, dtype=object)
+    s1 = np.asarray(f[:, 0], dtype=np.int64)
+    m1 = np.asarray(f[:, 1], dtype=np.int64)
+    c1 = np.asarray(f[:, 2], dtype=np.float64)
+    g = np.asarray(second, dtype=object)
+    m2 = np.asarray(g[:, 0], dtype=np.int64)
+    d2 = np.asarray(g[:, 1], dtype=np.int64)
+    c2 = np.asarray(g[:, 2], dtype=np.float64)
+    order = np.argsort(m2, kind="stable")
+    m2s = m2[order]
+    lo = np.searchsorted(m2s, m1, side="left")
+    hi = np.searchsorted(m2s, m1, side="right")
+    fan = hi - lo
+    if not fan.any():
+        return ()
+    # expand: term k of `first` pairs with second's terms order[lo[k] : hi[k]]
+    rep = np.repeat(np.arange(len(m1)), fan)
+    within = np.arange(int(fan.sum())) - np.repeat(np.cumsum(fan) - fan, fan)
+    j = order[lo[rep] + within]
+    src, dst, w = s1[rep], d2[j], c1[rep] * c2[j]
+    # merge duplicate (src, dst) pairs, keeping the order of their first appearance
+    width = int(dst.max()) + 1
+    key = src * width + dst
+    uniq, first_at, inverse = np.unique(key, return_index=True, return_inverse=True)
+    total = np.zeros(len(uniq))
+    for pos in range(len(key)):  # sequential, so the summation order is the path order
+        total[inverse[pos]] += w[pos]
+    seq = np.argsort(first_at, kind="stable")
+    return tuple((int(uniq[i] // width), int(uniq[i] % width), float(total[i])) for i in seq)
+
+
+def _tree_map(structure: TensorStructure, new: TensorStructure, expand) -> tuple:
+    """Block map whose terms come from ``expand(key) -> ((new_key, coeff), ...)``; exact
+    zero coefficients are not recorded."""
+    where = new.index_of
+    return tuple((src, where(nk), c) for src, key in enumerate(structure.block_order)
+                 for nk, c in expand(key) if c != 0)
+
+
+# ----------------------------------------------------------------------------- permutation
 
 
 @dataclass(frozen=True)
@@ -62,80 +143,78 @@ def validated_axes(ndim: int, axes) -> tuple:
     return axes
 
 
-def _side_perm(axes, old_side, new_side) -> tuple:
-    """Permutation induced on one side's legs: entry j = old in-side position of the leg
-    that becomes in-side position j."""
-    pos = {ax: k for k, ax in enumerate(old_side)}
-    return tuple(pos[axes[new_ax]] for new_ax in new_side)
+def _within_side(axes, old_axes, new_axes) -> tuple:
+    """The reordering a public permutation induces on one side: entry j is the old side
+    position of the leg that ends up at side position j."""
+    rank_on_side = dict(zip(old_axes, range(len(old_axes))))
+    return tuple(rank_on_side[axes[a]] for a in new_axes)
 
 
-@cache
-def permutation_plan(structure: TensorStructure, axes: tuple) -> PermutationPlan:
-    plan = _pattern_permutation(pattern_of(structure), axes)
-    legs = tuple(structure.legs[i] for i in axes)
+def _with_legs(plan, structure: TensorStructure, legs: tuple):
+    """A pattern-level plan re-targeted at the real legs (same block order, real shapes)."""
     if legs == plan.new_structure.legs:
         return plan
     return replace(plan, new_structure=TensorStructure(legs))
 
 
 @cache
-def _pattern_permutation(structure: TensorStructure, axes: tuple) -> PermutationPlan:
-    new = TensorStructure(tuple(structure.legs[i] for i in axes))
-    out_perm = _side_perm(axes, structure.out_axes, new.out_axes)
-    in_perm = _side_perm(axes, structure.in_axes, new.in_axes)
-    if out_perm == tuple(range(len(out_perm))) and in_perm == tuple(range(len(in_perm))):
-        # Only the OUT/IN interleaving moved: every key survives unchanged and the
-        # sorted block order is identical (reference permutation.py:166-170).
-        return PermutationPlan(axes, new, tuple((i, i, 1.0) for i in range(structure.num_blocks)))
-    provider = structure.provider
+def permutation_plan(structure: TensorStructure, axes: tuple) -> PermutationPlan:
+    """Public axis ``j`` of the result is axis ``axes[j]`` of ``structure``."""
+    legs = tuple(structure.legs[a] for a in axes)
+    return _with_legs(_permutation_on_pattern(pattern_of(structure), axes), structure, legs)
+
+
+@cache
+def _permutation_on_pattern(pattern: TensorStructure, axes: tuple) -> PermutationPlan:
+    new = TensorStructure(tuple(pattern.legs[a] for a in axes))
+    on_out = _within_side(axes, pattern.out_axes, new.out_axes)
+    on_in = _within_side(axes, pattern.in_axes, new.in_axes)
+    if on_out == tuple(sorted(on_out)) and on_in == tuple(sorted(on_in)):
+        # only the interleaving of OUT and IN legs changed: the trees, hence the keys and
+        # their sorted order, are untouched
+        return PermutationPlan(axes, new, identity_map(pattern.num_blocks))
+    provider = pattern.provider
     if not hasattr(provider, "permute_tree"):
         raise CapabilityError(f"{provider.name} does not implement permutation coefficients")
-    terms = []
-    for src, key in enumerate(structure.block_order):
-        for ot, c_out in provider.permute_tree(key.output_tree, out_perm):
-            for it, c_in in provider.permute_tree(key.input_tree, in_perm):
-                coeff = c_out * c_in
-                if coeff == 0:
-                    continue
-                terms.append((src, new.index_of(FusionBlockKey(ot, it)), coeff))
-    return PermutationPlan(axes, new, tuple(terms))
+    # each distinct tree of a side is expanded once
+    out_exp = {t: provider.permute_tree(t, on_out) for t in {k.output_tree for k in pattern.block_order}}
+    in_exp = {t: provider.permute_tree(t, on_in) for t in {k.input_tree for k in pattern.block_order}}
+
+    def expand(key):
+        return [(FusionBlockKey(ot, it), co * ci)
+                for ot, co in out_exp[key.output_tree] for it, ci in in_exp[key.input_tree]]
+
+    return PermutationPlan(axes, new, _tree_map(pattern, new, expand))
 
 
 # ----------------------------------------------------------------------------- bend
 
 
-def _on_side(leg: Leg, side: Side) -> Leg:
-    return leg if leg.side is side else replace(leg, side=side, dual=not leg.dual)
+def _flipped(leg: Leg) -> Leg:
+    """The same wire seen from the other side of the map: side swapped, dual toggled."""
+    return replace(leg, side=IN if leg.side is OUT else OUT, dual=not leg.dual)
+
+
+def _bent_legs(legs: tuple, axis: int) -> tuple:
+    return (*legs[:axis], *legs[axis + 1:], _flipped(legs[axis]))
 
 
 @cache
 def bend_plan(structure: TensorStructure, axis: int) -> PermutationPlan:
-    """Bend ``axis`` (the last leg of its side) to the end of the other side."""
-    plan = _pattern_bend(pattern_of(structure), axis)
-    leg = structure.legs[axis]
-    moved = _on_side(leg, IN if leg.side is OUT else OUT)
-    legs = (*(o for i, o in enumerate(structure.legs) if i != axis), moved)
-    if legs == plan.new_structure.legs:
-        return plan
-    return replace(plan, new_structure=TensorStructure(legs))
+    """Bend ``axis`` -- the last leg of its side -- to the end of the other side."""
+    return _with_legs(_bend_on_pattern(pattern_of(structure), axis), structure,
+                      _bent_legs(structure.legs, axis))
 
 
 @cache
-def _pattern_bend(structure: TensorStructure, axis: int) -> PermutationPlan:
-    provider = structure.provider
-    leg = structure.legs[axis]
-    right = leg.side is OUT
-    bend = provider.bend_right if right else provider.bend_left
-    moved = replace(leg, side=IN if right else OUT, dual=not leg.dual)
-    new = TensorStructure((*(o for i, o in enumerate(structure.legs) if i != axis), moved))
-    terms = []
-    for src, key in enumerate(structure.block_order):
-        for new_key, coeff in bend(key, dual=leg.dual):
-            if coeff == 0:
-                continue
-            terms.append((src, new.index_of(new_key), coeff))
-    perm = (*(i for i in range(structure.ndim) if i != axis), axis)
-    return PermutationPlan(perm, new, tuple(terms))
+def _bend_on_pattern(pattern: TensorStructure, axis: int) -> PermutationPlan:
+    leg = pattern.legs[axis]
+    provider = pattern.provider
+    move = provider.bend_right if leg.side is OUT else provider.bend_left
+    new = TensorStructure(_bent_legs(pattern.legs, axis))
+    terms = _tree_map(pattern, new, lambda key: move(key, dual=leg.dual))
+    order = tuple(a for a in range(pattern.ndim) if a != axis) + (axis,)
+    return PermutationPlan(order, new, terms)
 
 
 # ----------------------------------------------------------------------------- repartition
@@ -143,7 +222,7 @@ def _pattern_bend(structure: TensorStructure, axis: int) -> PermutationPlan:
 
 @dataclass(frozen=True)
 class RepartitionPlan:
-    """One composed block permutation ``perm`` plus the composed sparse block map."""
+    """One composed axis permutation ``perm`` plus the composed block map."""
 
     new_structure: TensorStructure
     perm: tuple
@@ -151,26 +230,8 @@ class RepartitionPlan:
 
     @property
     def trivial(self) -> bool:
-        """True when every block maps one-to-one with coefficient 1 and no axis moves."""
-        return _is_identity_map(self.terms) and self.perm == tuple(range(len(self.perm)))
-
-
-def _is_identity_map(terms) -> bool:
-    return all(s == d and c == 1 for s, d, c in terms)
-
-
-def _compose_terms(first, second) -> tuple:
-    """Sparse product of two block maps, duplicate (src, dst) pairs summed in first-seen
-    order (reference ``repartition.py:298-316``)."""
-    by_src: dict = {}
-    for s, d, c in second:
-        by_src.setdefault(s, []).append((d, c))
-    merged: dict = {}
-    for s, mid, c in first:
-        for d, c2 in by_src.get(mid, ()):
-            k = (s, d)
-            merged[k] = c * c2 if k not in merged else merged[k] + c * c2
-    return tuple((s, d, c) for (s, d), c in merged.items())
+        """True when every block maps to itself with weight 1 and no axis moves."""
+        return self.perm == tuple(range(len(self.perm))) and all(s == d and c == 1 for s, d, c in self.terms)
 
 
 def validated_partition(ndim: int, outputs, inputs) -> tuple:
@@ -199,47 +260,51 @@ def validated_partition(ndim: int, outputs, inputs) -> tuple:
     return tuple(parts)
 
 
+def _target_sides(outputs: tuple, inputs: tuple) -> dict:
+    return {**{a: OUT for a in outputs}, **{a: IN for a in inputs}}
+
+
 @cache
 def repartition_plan(structure: TensorStructure, outputs: tuple, inputs: tuple) -> RepartitionPlan:
-    """Plan ``repartition``: outputs become OUT, inputs IN, public order ``(*outputs, *inputs)``."""
-    want = {ax: OUT for ax in outputs} | {ax: IN for ax in inputs}
-    if not any(structure.legs[ax].side is not want[ax] for ax in range(structure.ndim)):
+    """``outputs`` become OUT and ``inputs`` IN, public order ``(*outputs, *inputs)``."""
+    side = _target_sides(outputs, inputs)
+    if all(structure.legs[a].side is side[a] for a in range(structure.ndim)):
         p = permutation_plan(structure, (*outputs, *inputs))
         return RepartitionPlan(p.new_structure, p.axes, p.terms)
-    plan = _pattern_repartition(pattern_of(structure), outputs, inputs)
-    legs = tuple(_on_side(structure.legs[ax], want[ax]) for ax in (*outputs, *inputs))
-    if legs == plan.new_structure.legs:
-        return plan
-    return replace(plan, new_structure=TensorStructure(legs))
+    legs = tuple(leg if leg.side is side[a] else _flipped(leg)
+                 for a, leg in ((a, structure.legs[a]) for a in (*outputs, *inputs)))
+    return _with_legs(_repartition_on_pattern(pattern_of(structure), outputs, inputs), structure, legs)
+
+
+def _moves(sides: tuple, outputs: tuple, inputs: tuple) -> list:
+    """The elementary moves of a repartition, from the legs' current sides alone:
+    ``("permute", axes)`` and ``("bend", position)``.  Every axis that changes side, in
+    ascending order, is carried to the last public position and bent there; a final
+    permutation lists the axes as ``(*outputs, *inputs)``."""
+    want = _target_sides(outputs, inputs)
+    n = len(sides)
+    at = list(range(n))  # at[p] = original axis now at public position p
+    moves = []
+    for a in range(n):
+        if sides[a] is want[a]:
+            continue
+        p = at.index(a)
+        moves.append(("permute", (*range(p), *range(p + 1, n), p)))
+        at.append(at.pop(p))
+        moves.append(("bend", n - 1))
+    moves.append(("permute", tuple(at.index(a) for a in (*outputs, *inputs))))
+    return moves
 
 
 @cache
-def _pattern_repartition(structure: TensorStructure, outputs: tuple, inputs: tuple):
-    """Walk transpose -> bend per crossing leg -> final transpose, composing the block maps
-    and the axis permutations (reference ``repartition.py:341-385``)."""
-    ndim = structure.ndim
-    want = {ax: OUT for ax in outputs} | {ax: IN for ax in inputs}
-    crossing = [ax for ax in range(ndim) if structure.legs[ax].side is not want[ax]]
-    perm = tuple(range(ndim))
-    terms = tuple((i, i, 1.0) for i in range(structure.num_blocks))
-    labels = list(range(ndim))
-    for ax in crossing:
-        p = labels.index(ax)
-        axes = tuple(i for i in range(ndim) if i != p) + (p,)
-        tp = permutation_plan(structure, axes)
-        perm = tuple(perm[i] for i in axes)
-        terms = _compose_terms(terms, tp.terms)
-        structure = tp.new_structure
-        labels.append(labels.pop(p))
-        bp = bend_plan(structure, ndim - 1)
-        terms = _compose_terms(terms, bp.terms)
-        structure = bp.new_structure
-    position = {a: p for p, a in enumerate(labels)}
-    axes = tuple(position[ax] for ax in (*outputs, *inputs))
-    tp = permutation_plan(structure, axes)
-    return RepartitionPlan(
-        tp.new_structure, tuple(perm[i] for i in axes), _compose_terms(terms, tp.terms)
-    )
+def _repartition_on_pattern(pattern: TensorStructure, outputs: tuple, inputs: tuple) -> RepartitionPlan:
+    cur, perm, terms = pattern, tuple(range(pattern.ndim)), identity_map(pattern.num_blocks)
+    for kind, arg in _moves(tuple(l.side for l in pattern.legs), outputs, inputs):
+        step = permutation_plan(cur, arg) if kind == "permute" else bend_plan(cur, arg)
+        perm = tuple(perm[i] for i in step.axes)
+        terms = compose_maps(terms, step.terms)
+        cur = step.new_structure
+    return RepartitionPlan(cur, perm, terms)
 
 
 # ----------------------------------------------------------------------------- map layout
@@ -273,58 +338,48 @@ class MapLayout:
         )
 
 
+def _tree_extent(structure: TensorStructure, axes: tuple, tree) -> int:
+    """Rows (or columns) a tree's block occupies: the product of the degeneracies of its
+    uncoupled sectors on the legs of that side."""
+    return int(np.prod([structure.legs[a].degeneracy(structure.legs[a].space_sector(u))
+                        for a, u in zip(axes, tree.uncoupled)], dtype=np.int64))
+
+
 @cache
 def map_layout(structure: TensorStructure) -> MapLayout:
-    out_axes, in_axes = structure.out_axes, structure.in_axes
-    row_trees: dict = {}
-    col_trees: dict = {}
-    for key in structure.block_order:
-        row_trees.setdefault(key.coupled, {})[key.output_tree] = None
-        col_trees.setdefault(key.coupled, {})[key.input_tree] = None
-    sectors = tuple(sorted(row_trees))
+    """Blocks of one coupled sector tile that sector's matrix: OUT trees index row bands,
+    IN trees column bands, both in order of first appearance in the block order (which
+    is sorted, so the bands are too).  Every (row tree, column tree) pair of a sector must
+    be a block, or the structure is not a per-sector cross product."""
+    per_sector: dict = {}  # c -> ({out tree: band}, {in tree: band}, {(band, band): block})
+    for i, key in enumerate(structure.block_order):
+        r, k, cells = per_sector.setdefault(key.coupled, ({}, {}, {}))
+        cells[(r.setdefault(key.output_tree, len(r)), k.setdefault(key.input_tree, len(k)))] = i
     rows, cols, grid = [], [], []
-    for c in sectors:
-        rb = _bands(structure, out_axes, c, row_trees[c])
-        cb = _bands(structure, in_axes, c, col_trees[c])
-        rows.extend(rb)
-        cols.extend(cb)
-        grid.append(
-            (
-                c,
-                tuple(
-                    tuple(structure.index_of(FusionBlockKey(ot, it)) for _, it, _, _ in cb)
-                    for _, ot, _, _ in rb
-                ),
-            )
-        )
-    covered = sum(len(cells) * len(cells[0]) for _, cells in grid)
-    if covered != structure.num_blocks:
-        raise RuntimeError("map_layout: block order is not a per-sector cross product")
-    return MapLayout(structure, sectors, tuple(rows), tuple(cols), tuple(grid))
-
-
-def _bands(structure, axes, c, trees) -> tuple:
-    bands, offset = [], 0
-    for tree in trees:
-        extent = 1
-        for ax, u in zip(axes, tree.uncoupled):
-            leg = structure.legs[ax]
-            extent *= leg.degeneracy(leg.space_sector(u))
-        bands.append((c, tree, offset, extent))
-        offset += extent
-    return tuple(bands)
+    for c in sorted(per_sector):
+        r, k, cells = per_sector[c]
+        if len(cells) != len(r) * len(k):
+            raise RuntimeError("map_layout: block order is not a per-sector cross product")
+        for bands, axes, dst in ((r, structure.out_axes, rows), (k, structure.in_axes, cols)):
+            extents = [_tree_extent(structure, axes, t) for t in bands]
+            starts = np.concatenate([[0], np.cumsum(extents)[:-1]]).astype(np.int64) if extents else []
+            dst.extend((c, t, int(o), int(e)) for t, o, e in zip(bands, starts, extents))
+        grid.append((c, tuple(tuple(cells[(a, b)] for b in range(len(k))) for a in range(len(r)))))
+    return MapLayout(structure, tuple(sorted(per_sector)), tuple(rows), tuple(cols), tuple(grid))
 
 
 # ----------------------------------------------------------------------------- contraction
 
 
 def outward_dual(leg: Leg) -> bool:
-    """The all-outgoing object of a leg: dualized exactly when ``side is IN`` xor ``dual``."""
-    return leg.dual != (leg.side is IN)
+    """Whether the wire, read as leaving the tensor, carries the dual space: exactly when
+    the leg sits on the domain side xor is marked dual."""
+    return (leg.side is IN) != leg.dual
 
 
 def contractible(x: Leg, y: Leg) -> bool:
-    return x.space == y.space and outward_dual(x) != outward_dual(y)
+    """Two legs join into one wire when they carry the same space and point opposite ways."""
+    return x.space == y.space and outward_dual(x) is not outward_dual(y)
 
 
 @dataclass(frozen=True)
@@ -378,40 +433,38 @@ def validated_contraction_axes(a: TensorStructure, b: TensorStructure, axes) -> 
     return tuple(parts)
 
 
+def _refusal(k: int, i: int, j: int, x: Leg, y: Leg) -> ValueError:
+    why = ("their spaces differ" if x.space != y.space
+           else "both ends of the wire point the same way (equal outward duals)")
+    return ValueError(f"tensordot: pair {k}, axis {i} of a and axis {j} of b do not contract: {why}")
+
+
 @cache
 def contraction_plan(a: TensorStructure, b: TensorStructure, axes: tuple) -> ContractionPlan:
+    """Lower ``tensordot(a, b, axes)`` to ``a: (free | contracted)``, ``b: (contracted |
+    free)``, one composition, and a restoring repartition of the composed legs: each free
+    leg returns to its own side, public order = a's free legs then b's."""
     ca, cb = axes
     if a.provider != b.provider:
-        raise ValueError(
-            f"tensordot: a has provider {a.provider.name} and b has {b.provider.name}"
-        )
-    for k, (i, j) in enumerate(zip(ca, cb)):
-        x, y = a.legs[i], b.legs[j]
-        if not contractible(x, y):
-            reason = (
-                "their spaces differ"
-                if x.space != y.space
-                else "both ends of the wire point the same way (equal outward duals)"
-            )
-            raise ValueError(
-                f"tensordot: pair {k}, axis {i} of a and axis {j} of b do not contract: {reason}"
-            )
-    sca, scb = set(ca), set(cb)
-    fa = tuple(i for i in range(a.ndim) if i not in sca)
-    fb = tuple(j for j in range(b.ndim) if j not in scb)
-    if not fa and not fb:
+        raise ValueError(f"tensordot: a has provider {a.provider.name} and b has {b.provider.name}")
+    bad = next(((k, i, j) for k, (i, j) in enumerate(zip(ca, cb)) if not contractible(a.legs[i], b.legs[j])), None)
+    if bad is not None:
+        k, i, j = bad
+        raise _refusal(k, i, j, a.legs[i], b.legs[j])
+    free_a = tuple(sorted(set(range(a.ndim)) - set(ca)))
+    free_b = tuple(sorted(set(range(b.ndim)) - set(cb)))
+    if not free_a and not free_b:
         raise ValueError("tensordot: this contraction leaves no free leg")
-    free = (*(a.legs[i] for i in fa), *(b.legs[j] for j in fb))
-    outputs = tuple(k for k, leg in enumerate(free) if leg.side is OUT)
-    inputs = tuple(k for k, leg in enumerate(free) if leg.side is IN)
-    position = {k: p for p, k in enumerate((*outputs, *inputs))}
+    legs = tuple(a.legs[i] for i in free_a) + tuple(b.legs[j] for j in free_b)
+    sides = np.array([leg.side is OUT for leg in legs], dtype=bool)
+    outs = tuple(int(k) for k in np.flatnonzero(sides))
+    ins = tuple(int(k) for k in np.flatnonzero(~sides))
+    # the composed tensor lists (*outs, *ins); the last transpose puts each leg back
+    where = np.empty(len(legs), dtype=np.int64)
+    where[list(outs + ins)] = np.arange(len(legs))
     return ContractionPlan(
-        a_outputs=fa,
-        a_inputs=ca,
-        b_outputs=cb,
-        b_inputs=fb,
-        restore_outputs=outputs,
-        restore_inputs=inputs,
-        final_transpose=tuple(position[k] for k in range(len(free))),
-        new_structure=TensorStructure(free),
+        a_outputs=free_a, a_inputs=ca, b_outputs=cb, b_inputs=free_b,
+        restore_outputs=outs, restore_inputs=ins,
+        final_transpose=tuple(int(x) for x in where),
+        new_structure=TensorStructure(legs),
     )

@@ -141,22 +141,32 @@ _ONE_Z = np.ones((1, 1))
 _ONE_Z.flags.writeable = False
 
 
-def _bend_unique(provider, key, right: bool):
-    from tenet.fusion_tree import coupled_sectors
+def _bent_key(provider, key, right: bool):
+    """The block key after bending the last uncoupled line of one tree onto the end of the
+    other: ``right`` takes it from the output tree, else from the input tree.  The source
+    vertex is ``a x b -> c`` (``a`` the unit for a rank-1 source tree); the shortened source
+    tree keeps its spine up to ``a``, and the other tree grows by ``dual(b)`` fusing to
+    ``a``.  Returns ``(new key, a, b, c)``."""
+    from tenet.fusion_tree import FusionTree
     from tenet.structure import FusionBlockKey
 
-    src = key.output_tree if right else key.input_tree
-    dst = key.input_tree if right else key.output_tree
+    src, dst = (key.output_tree, key.input_tree) if right else (key.input_tree, key.output_tree)
     if src.rank == 0:
         raise ValueError(f"{provider.name}: cannot bend from an empty tree of {key}")
-    moved, remaining = src.uncoupled[-1], src.uncoupled[:-1]
-    coupled = coupled_sectors(provider, remaining)
-    if len(coupled) != 1:
-        raise CapabilityError(f"{provider.name}: {remaining} does not fuse uniquely")
-    new_src = _only_tree(provider, remaining, coupled[0])
-    new_dst = _only_tree(provider, (*dst.uncoupled, provider.dual(moved)), coupled[0])
-    new_key = FusionBlockKey(new_src, new_dst) if right else FusionBlockKey(new_dst, new_src)
-    return ((new_key, 1.0),)
+    b, c = src.uncoupled[-1], src.coupled
+    spine = src.lines()
+    a = spine[-2] if src.rank >= 2 else provider.unit
+    shorter = FusionTree(src.uncoupled[:-1], src.inner[:-1], src.multiplicities[:-1], a)
+    longer = FusionTree((*dst.uncoupled, provider.dual(b)), dst.lines()[1:],
+                        (*dst.multiplicities, 0) if dst.rank else (), a)
+    new_key = FusionBlockKey(shorter, longer) if right else FusionBlockKey(longer, shorter)
+    return new_key, a, b, c
+
+
+def _bend_unique(provider, key, right: bool):
+    """Unique fusion (Trivial, U(1), fZ2): the bent key with coefficient exactly 1
+    (reference ``bend_unique``, ``symmetry/base.py:999-1074``)."""
+    return ((_bent_key(provider, key, right)[0], 1.0),)
 
 
 @dataclass(frozen=True, slots=True)
@@ -402,91 +412,86 @@ class FZ2Provider:
 # --------------------------------------------------------------------------------------
 
 
+def _adjacent_swaps(perm) -> list:
+    """A reduced word for ``perm`` (``perm[j]`` = old position that ends at ``j``) as the
+    positions of adjacent exchanges, applied left to right.  Positions are filled from the
+    right: the line that belongs last is carried rightwards to its place, then the next."""
+    cur = list(range(len(perm)))
+    word = []
+    for j in range(len(perm) - 1, -1, -1):
+        p = cur.index(perm[j])
+        for i in range(p, j):
+            word.append(i)
+            cur[i], cur[i + 1] = cur[i + 1], cur[i]
+    return word
+
+
 @cache
 def _permute_braided(provider, tree, perm):
-    """Expand ``tree`` with uncoupled lines permuted by ``perm`` into a tree sum.
-
-    ``perm[j]`` is the old position that becomes position ``j``.  The permutation is
-    bubble-sorted into adjacent swaps applied in order (reference ``base.py:884-895``);
-    each swap is one Artin generator.  Returns ``((tree, coeff), ...)`` in first-seen
-    order of the accumulation.
-    """
-    terms: dict = {tree: 1.0}
-    current = list(range(tree.rank))
-    for j, target in enumerate(perm):
-        for i in range(current.index(target) - 1, j - 1, -1):
-            current[i], current[i + 1] = current[i + 1], current[i]
-            nxt: dict = {}
-            for t, c in terms.items():
-                for braided, cb in _swap(provider, t, i):
-                    nxt[braided] = nxt.get(braided, 0.0) + c * cb
-            terms = nxt
-    return tuple(terms.items())
+    """``tree`` with its uncoupled lines permuted by ``perm`` (``perm[j]`` = old position of
+    the line at new position ``j``), expanded in the tree basis: the product of one braid
+    generator per exchange of a reduced word for ``perm``.  For a symmetric braiding such
+    as SU(2)'s (``R^2 = 1``) every reduced word gives the same map, so the word need not be
+    the reference's (``permute_braided_tree``, ``symmetry/base.py:829-895``).  Returns
+    ``((tree, coeff), ...)``."""
+    state = {tree: 1.0}
+    for i in _adjacent_swaps(perm):
+        nxt: dict = {}
+        for t, w in state.items():
+            for t2, g in _braid_generator(provider, t, i):
+                nxt[t2] = nxt.get(t2, 0.0) + w * g
+        state = nxt
+    return tuple(state.items())
 
 
 @cache
-def _swap(provider, tree, i: int):
-    """Exchange uncoupled lines ``i`` and ``i+1`` of a left-associated tree.
+def _braid_generator(provider, tree, i: int):
+    """The braid generator exchanging uncoupled lines ``i`` and ``i + 1`` of a
+    left-associated tree, as ``((tree, coeff), ...)``.
 
-    ``i == 0``: one R-move on the first vertex.  ``i >= 1``: with ``a = e_{i-1}``,
-    ``b = u_i``, ``c = u_{i+1}``, ``f = e_{i+1}`` and old inner line ``e``,
-    ``coeff(e -> e') = sum_d F^{abc}_f[e,d] R^{bc}_d F^{acb}_f[e',d]`` (real gauge).
-    Reference ``_artin_braid``, ``symmetry/base.py:898-996``.
+    ``i == 0``: the two lines meet at the first vertex; one R-symbol
+    ``R^{u0 u1}_{e1}``.  ``i >= 1``: the local subtree is ``((a x b)_e x c)_f`` with
+    ``a = e_{i-1}``, ``b = u_i``, ``c = u_{i+1}``, ``f = e_{i+1}``.  An F-move reassociates
+    it to ``(a x (b x c)_d)_f`` (row ``e`` of ``F^{abc}_f``), the R-symbol ``R^{bc}_d``
+    exchanges ``b`` and ``c`` inside ``d``, and the inverse F-move of the exchanged triple
+    (``F^{acb}_f`` transposed, F being real orthogonal in this gauge) returns to the
+    left-associated basis with a new inner line ``e'``.
     """
     from tenet.fusion_tree import FusionTree
 
     u = tree.uncoupled
     spine = tree.lines()
-    new_u = (*u[:i], u[i + 1], u[i], *u[i + 2 :])
+    exchanged = (*u[:i], u[i + 1], u[i], *u[i + 2:])
     if i == 0:
-        swapped = FusionTree(new_u, tree.inner, tree.multiplicities, tree.coupled)
-        return ((swapped, provider.r_symbol(u[0], u[1], spine[1])),)
+        return ((FusionTree(exchanged, tree.inner, tree.multiplicities, tree.coupled),
+                 provider.r_symbol(u[0], u[1], spine[1])),)
     a, b, c, e, f = spine[i - 1], u[i], u[i + 1], spine[i], spine[i + 1]
-    out = []
-    for new_e in provider.fusion(a, c):
-        if not provider.n_symbol(new_e, b, f):
+    mids = [d for d in provider.fusion(b, c) if provider.n_symbol(a, d, f)]
+    # F^{abc}_f[e, d] R^{bc}_d: the reassociated, exchanged amplitudes over the middle line d
+    amp = {d: provider.f_symbol(a, b, c, f, e, d) * provider.r_symbol(b, c, d) for d in mids}
+    terms = []
+    for e2 in provider.fusion(a, c):
+        if not provider.n_symbol(e2, b, f):
             continue
-        inner = (*tree.inner[: i - 1], new_e, *tree.inner[i:])
-        coeff = 0.0
-        for d in provider.fusion(b, c):
-            coeff += (
-                provider.f_symbol(a, b, c, f, e, d)
-                * provider.r_symbol(b, c, d)
-                * provider.f_symbol(a, c, b, f, new_e, d)
-            )
-        out.append((FusionTree(new_u, inner, tree.multiplicities, tree.coupled), coeff))
-    return tuple(out)
+        w = 0.0
+        for d in mids:
+            w += amp[d] * provider.f_symbol(a, c, b, f, e2, d)
+        inner = (*tree.inner[: i - 1], e2, *tree.inner[i:])
+        terms.append((FusionTree(exchanged, inner, tree.multiplicities, tree.coupled), w))
+    return tuple(terms)
 
 
 def _bend_braided(provider, key, right: bool, dual: bool):
-    """Move the last line of one tree to the end of the other.
-
-    Source vertex ``a x b -> c`` (``a`` the unit for a rank-1 source); the coefficient is
-    ``sqrt(qdim(c)/qdim(a)) * B(a,b,c)``, times ``chi(dual(b))`` when the moved leg was
-    already dual; ``bend_left`` reads the same expression off the input tree (its
-    conjugate, equal for a real gauge).  Reference ``bend_braided``, ``base.py:1077-1190``.
-    """
-    from tenet.fusion_tree import FusionTree
-    from tenet.structure import FusionBlockKey
-
-    src = key.output_tree if right else key.input_tree
-    dst = key.input_tree if right else key.output_tree
-    if src.rank == 0:
-        raise ValueError(f"{provider.name}: cannot bend from an empty tree of {key}")
-    b, c = src.uncoupled[-1], src.coupled
-    a = src.lines()[-2] if src.rank >= 2 else provider.unit
-    new_src = FusionTree(src.uncoupled[:-1], src.inner[:-1], src.multiplicities[:-1], a)
-    scale = sqrt(provider.qdim(c) / provider.qdim(a))
+    """Bend for SU(2): the bent key of ``_bent_key`` with coefficient
+    ``sqrt(qdim(c) / qdim(a)) * B(a, b, c)``, times the Frobenius-Schur indicator of
+    ``dual(b)`` when the moved leg was already dual (the input-tree reading is the
+    conjugate, equal in this real gauge).  Reference ``bend_braided``,
+    ``symmetry/base.py:1077-1190``."""
+    new_key, a, b, c = _bent_key(provider, key, right)
+    w = sqrt(provider.qdim(c) / provider.qdim(a)) * provider.b_symbol(a, b, c)
     if dual:
-        scale *= provider.frobenius_schur(provider.dual(b))
-    new_dst = FusionTree(
-        (*dst.uncoupled, provider.dual(b)),
-        dst.lines()[1:],
-        (*dst.multiplicities, 0) if dst.rank else (),
-        a,
-    )
-    new_key = FusionBlockKey(new_src, new_dst) if right else FusionBlockKey(new_dst, new_src)
-    return ((new_key, scale * provider.b_symbol(a, b, c)),)
+        w *= provider.frobenius_schur(provider.dual(b))
+    return ((new_key, w),)
 
 
 Trivial = TrivialProvider()
