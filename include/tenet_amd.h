@@ -195,6 +195,28 @@ int tn_tridiag_ground(const double* alpha, const double* beta, int n, double tol
 int tn_tridiag_ground_host(const double* alpha, const double* beta, int n, double tol,
                            double* value, double* coef);
 
+/* Many independent problems in one arena (the batched Lanczos of many H_eff problems,
+ * reference network/dmrg.py:106-227 run for each): problem s owns the elements
+ * [off_s, off_{s+1}).  `chunks` lists (segment, begin, end) int64 triples that cover every
+ * problem's range in pieces of bounded size; seg_chunks[s] .. seg_chunks[s+1] are problem
+ * s's chunks, in order.
+ * tn_dot_segmented: out[s * out_stride] = sum_i w_i x_i y_i over problem s (w may be NULL),
+ * FP64, deterministic (one partial per chunk into `partial` (nchunks doubles), then the
+ * partials of a problem summed in chunk order).
+ * tn_axpby_segmented: y = (a * A_s) x + (b * B_s) y with A_s = a_dev[s * a_stride] (or
+ * its inverse when a_inv; 1 when a_dev is NULL), likewise B_s; a zero factor drops its
+ * operand unread, as tn_axpby.
+ * tn_tridiag_ground_batched: tn_tridiag_ground for nprob problems in one launch, problem p's
+ * matrix at alpha + p ld / beta + p ld, its vector at coefs + p ld, its value at values[p]. */
+int tn_dot_segmented(int dtype, const void* x, const void* y, const double* w, const int64_t* chunks,
+                     int64_t nchunks, double* partial, const int64_t* seg_chunks, int64_t nseg,
+                     double* out, int64_t out_stride, void* stream);
+int tn_axpby_segmented(int dtype, const int64_t* chunks, int64_t nchunks, double a, const double* a_dev,
+                       int64_t a_stride, int a_inv, const void* x, double b, const double* b_dev,
+                       int64_t b_stride, int b_inv, void* y, void* stream);
+int tn_tridiag_ground_batched(const double* alpha, const double* beta, int n, int64_t ld, int64_t nprob,
+                              double tol, double* values, double* coefs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

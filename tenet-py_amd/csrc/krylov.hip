@@ -163,10 +163,8 @@ inline int tridiag_ground_serial(const double* a, const double* b, int n, double
 // points of the bracket, a ballot finds the first point with an eigenvalue below it, and the
 // bracket shrinks 65-fold per round (about nine rounds to the resolution of doubles).  The
 // vector then comes from lane 0.
-__global__ __launch_bounds__(64) void tridiag_ground_kernel(const double* __restrict__ alpha,
-                                                            const double* __restrict__ beta, int n,
-                                                            double tol, double* __restrict__ value,
-                                                            double* __restrict__ coef) {
+__device__ void tridiag_ground_device(const double* __restrict__ alpha, const double* __restrict__ beta, int n,
+                                      double tol, double* __restrict__ value, double* __restrict__ coef) {
   __shared__ double a[kMax], b[kMax], x[kMax];
   __shared__ TridiagWork w;
   __shared__ TridiagSetup st;
@@ -201,6 +199,24 @@ __global__ __launch_bounds__(64) void tridiag_ground_kernel(const double* __rest
   for (int i = t; i < n; i += 64) coef[i] = i < m ? x[i] : 0.0;
 }
 
+__global__ __launch_bounds__(64) void tridiag_ground_kernel(const double* __restrict__ alpha,
+                                                            const double* __restrict__ beta, int n,
+                                                            double tol, double* __restrict__ value,
+                                                            double* __restrict__ coef) {
+  tridiag_ground_device(alpha, beta, n, tol, value, coef);
+}
+
+// One workgroup per problem: problem p's diagonal at alpha + p ld, off-diagonal at
+// beta + p ld, its vector written to coef + p ld and its value to value[p].
+__global__ __launch_bounds__(64) void tridiag_batched_kernel(const double* __restrict__ alpha,
+                                                             const double* __restrict__ beta, int n,
+                                                             int64_t ld, double tol,
+                                                             double* __restrict__ value,
+                                                             double* __restrict__ coef) {
+  const int64_t p = blockIdx.x;
+  tridiag_ground_device(alpha + p * ld, beta + p * ld, n, tol, value + p, coef + p * ld);
+}
+
 }  // namespace
 
 extern "C" int tn_tridiag_ground(const double* alpha, const double* beta, int n, double tol,
@@ -220,4 +236,16 @@ extern "C" int tn_tridiag_ground_host(const double* alpha, const double* beta, i
   const int m = tridiag_ground_serial(alpha, b, n, tol, value, x, w);
   for (int i = 0; i < n; ++i) coef[i] = i < m ? x[i] : 0.0;
   return 0;
+}
+
+extern "C" int tn_tridiag_ground_batched(const double* alpha, const double* beta, int n, int64_t ld,
+                                         int64_t nprob, double tol, double* values, double* coefs,
+                                         void* stream) {
+  if (nprob <= 0) return 0;
+  if (n < 1 || n > kMax) return tn::fail("tn_tridiag_ground_batched: n must lie in [1, TN_TRIDIAG_MAX]");
+  if (ld < n) return tn::fail("tn_tridiag_ground_batched: leading dimension below n");
+  if (nprob > 0x7fffffffLL) return tn::fail("tn_tridiag_ground_batched: too many problems");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  tridiag_batched_kernel<<<static_cast<unsigned>(nprob), 64, 0, s>>>(alpha, beta, n, ld, tol, values, coefs);
+  return tn::check_launch("tn_tridiag_ground_batched");
 }

@@ -52,6 +52,9 @@ _SIGNATURES = {
     "tn_svd_jacobi_finish": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p]),
     "tn_tridiag_ground": (_i32, [_p, _p, _i32, _d, _p, _p, _p]),
     "tn_tridiag_ground_host": (_i32, [_p, _p, _i32, _d, _p, _p]),
+    "tn_tridiag_ground_batched": (_i32, [_p, _p, _i32, _i64, _i64, _d, _p, _p, _p]),
+    "tn_dot_segmented": (_i32, [_i32, _p, _p, _p, _p, _i64, _p, _p, _i64, _p, _i64, _p]),
+    "tn_axpby_segmented": (_i32, [_i32, _p, _i64, _d, _p, _i64, _i32, _p, _d, _p, _i64, _i32, _p, _p]),
 }
 TRIDIAG_MAX = 64
 

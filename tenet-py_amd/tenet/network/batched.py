@@ -244,22 +244,41 @@ class BatchedHeff2:
     def unpack(self, x: torch.Tensor) -> list:
         return [SymmetricTensor(s, x[int(a):int(b)]) for s, a, b in zip(self.structures, self.x_off[:-1], self.x_off[1:])]
 
+    def apply64(self, x: torch.Tensor) -> torch.Tensor:
+        """The batched matvec on an FP64 arena, returning an FP64 arena: the Krylov vectors
+        stay FP64 while the GEMMs run in the batch's arithmetic.  In the sector layout the
+        casts ride on the entry gather and the exit scatter; the block layout casts apart."""
+        if x.dtype != torch.float64 or x.numel() != self.x_size:
+            raise ValueError("BatchedHeff2.apply64: expects the FP64 arena of the batch")
+        if self.dtype == torch.float64:
+            return self.apply(x)
+        if self.layout == "sector":
+            return self._apply_sector(x, torch.float64)
+        from tenet import ops
+
+        return ops.cast_data(self.apply(ops.cast_data(x, self.dtype)), torch.float64)
+
+    def _apply_sector(self, x: torch.Tensor, out_dtype: torch.dtype, tiles: dict | None = None) -> torch.Tensor:
+        t = tiles or {}
+        wt = self.dtype
+        xs = torch.empty(max(self.xs_size, 1), dtype=wt, device=x.device)
+        self.entry.run(x, xs)
+        t1 = torch.empty(max(self.t1_size, 1), dtype=wt, device=x.device)
+        self.gemm1.run(xs, self.fr, t1, self.precision, tile=t.get("gemm1"))
+        t3 = torch.empty(max(self.t3_size, 1), dtype=wt, device=x.device)
+        self._mpo(t1, t3)
+        ws = torch.empty(max(self.gemm4.ws_size, 1), dtype=wt, device=x.device)
+        self.gemm4.run(self.fl, t3, ws, self.precision, tile=t.get("gemm4"))
+        out = torch.empty(self.x_size, dtype=out_dtype, device=x.device)
+        self.exit.run(ws, out)
+        return out
+
     def apply(self, x: torch.Tensor, tiles: dict | None = None) -> torch.Tensor:
         if x.dtype != self.dtype or x.numel() != self.x_size:
             raise ValueError("BatchedHeff2.apply: the arena does not match the batch")
         t = tiles or {}
         if self.layout == "sector":
-            xs = torch.empty(max(self.xs_size, 1), dtype=x.dtype, device=x.device)
-            self.entry.run(x, xs)
-            t1 = torch.empty(max(self.t1_size, 1), dtype=x.dtype, device=x.device)
-            self.gemm1.run(xs, self.fr, t1, self.precision, tile=t.get("gemm1"))
-            t3 = torch.empty(max(self.t3_size, 1), dtype=x.dtype, device=x.device)
-            self._mpo(t1, t3)
-            ws = torch.empty(max(self.gemm4.ws_size, 1), dtype=x.dtype, device=x.device)
-            self.gemm4.run(self.fl, t3, ws, self.precision, tile=t.get("gemm4"))
-            out = torch.empty(self.x_size, dtype=x.dtype, device=x.device)
-            self.exit.run(ws, out)
-            return out
+            return self._apply_sector(x, x.dtype, tiles)
         t1 = torch.empty(self.t1_size, dtype=x.dtype, device=x.device)
         self.gemm1.run(x, self.fr, t1, self.precision, tile=t.get("gemm1"))
         t3 = torch.empty(self.t3_size, dtype=x.dtype, device=x.device)
