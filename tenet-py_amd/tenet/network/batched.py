@@ -106,9 +106,21 @@ class BatchedHeff2:
     """One matvec over a batch of abelian two-site H_eff instances."""
 
     def __init__(self, instances, precision: str = "fp32", tiles: dict | None = None,
-                 layout: str = "sector") -> None:
+                 layout: str = "sector", operators=None) -> None:
+        """``instances``: ``(fl, w_left, w_right, fr, aa)`` per problem; or ``operators``:
+        ``(Heff2, structure)`` per problem, compiled operators reused as they are (their
+        FP32 copies and stacked environments are not rebuilt) -- the form a scheduler uses
+        to batch the current bonds of many running solves."""
         dt = torch.float64 if precision == "fp64" else torch.float32
         self.precision, self.dtype, self.layout = precision, dt, layout
+        if operators is not None:
+            if layout != "sector":
+                raise ValueError("batching compiled operators needs the sector layout")
+            for op, _ in operators:
+                if op.precision != precision or op.layout != "sector" or not op.fused:
+                    raise ValueError("every operator must be an abelian sector-layout Heff2 at the batch's precision")
+            self._init_sector(None, tiles or {}, operators)
+            return
         if layout == "sector":
             self._init_sector(instances, tiles or {})
             return
@@ -163,14 +175,18 @@ class BatchedHeff2:
                                 coef=np.asarray(coef, np.float64 if dt == torch.float64 else np.float32),
                                 tiles=tl.reshape(-1))
         self.flops = self.gemm1.flops + self.gemm4.flops
-        self.count = len(instances)
+        self.count = len(self.structures)
 
-    def _init_sector(self, instances, tune: dict) -> None:
+    def _init_sector(self, instances, tune: dict, operators=None) -> None:
         from tenet.engine import RecoupleProgram
         from tenet.network.heff import exit_regions
 
-        self.ops = [Heff2(fl, w1, w2, fr, self.precision, layout="sector") for fl, w1, w2, fr, _ in instances]
-        self.structures = [inst[4].structure for inst in instances]
+        if operators is None:
+            self.ops = [Heff2(fl, w1, w2, fr, self.precision, layout="sector") for fl, w1, w2, fr, _ in instances]
+            self.structures = [inst[4].structure for inst in instances]
+        else:
+            self.ops = [op for op, _ in operators]
+            self.structures = [s for _, s in operators]
         chains = [op.chain(s) for op, s in zip(self.ops, self.structures)]
         for ch in chains:
             if not hasattr(ch, "exit_spec"):
@@ -190,7 +206,11 @@ class BatchedHeff2:
         self.x_size = int(self.x_off[-1])
         self.xs_size, self.t1_size, self.t3_size = int(xs_off[-1]), int(t1_off[-1]), int(t3_off[-1])
         # stacked environments of every instance, once
-        stacks = [ch.prepare(op._datas) for op, ch in zip(self.ops, chains)]
+        stacks = []
+        for op, ch, st in zip(self.ops, chains, self.structures):
+            if st not in op._prepared:
+                op._prepared[st] = ch.prepare(op._datas)
+            stacks.append(op._prepared[st])
         self.fr = torch.cat([a for a, _ in stacks])
         self.fl = torch.cat([b for _, b in stacks])
         entry, p1, p3 = [], [], []
@@ -227,7 +247,7 @@ class BatchedHeff2:
                                 coef=np.asarray(coef, np.float64 if self.dtype == torch.float64 else np.float32),
                                 tiles=tl.reshape(-1))
         self.flops = self.gemm1.flops + self.gemm4.flops
-        self.count = len(instances)
+        self.count = len(self.structures)
 
     def _mpo(self, t1: torch.Tensor, t3: torch.Tensor) -> None:
         tb = self.mpo.get(t1.device)

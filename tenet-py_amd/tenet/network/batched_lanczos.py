@@ -116,15 +116,16 @@ class BatchedLanczos:
             self.r_b = torch.zeros(self.nseg, 2, **f64)
             self.r_g = torch.zeros(self.nseg, 2, **f64)
             self.r_u = torch.zeros(n, **f64)
-        # eager warm-up on random start vectors: compiles and uploads every table
-        self.V[0].normal_()
-        self.start_(self.V[0])
-        for j in range(ncv):
-            self._iteration(j)
-        self._finish()
-        torch.cuda.synchronize(dev)
         self.graphs = None
         if graphs:
+            # eager warm-up on random start vectors: compiles and uploads every table, so
+            # the captured work holds no host-to-device copy
+            self.V[0].normal_()
+            self.start_(self.V[0])
+            for j in range(ncv):
+                self._iteration(j)
+            self._finish()
+            torch.cuda.synchronize(dev)
             self.graphs = []
             pool = torch.cuda.graph_pool_handle()
             for step in [*range(ncv), "finish"]:

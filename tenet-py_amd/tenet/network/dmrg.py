@@ -28,7 +28,8 @@ import torch
 from tenet import linalg, ops
 from tenet.tensor import SymmetricTensor
 
-__all__ = ["Sweep", "DMRG_out", "lanczos", "sweep_", "sweep_steps", "dmrg_", "dmrg_steps", "spectrum"]
+__all__ = ["Sweep", "DMRG_out", "LanczosRequest", "lanczos", "sweep_", "sweep_steps", "dmrg_", "dmrg_steps",
+           "spectrum"]
 
 
 class Sweep(NamedTuple):
@@ -163,14 +164,25 @@ def _pair(left, right):
     return ops.einsum("apx,xqr->apqr", left, right)
 
 
-def _bond_step(psi, env, schmidt, n, forward, *, chi, cutoff, ncv, noise, seed, precision,
-               refine, orthogonal_to=()):
-    """One bond of a two-site sweep: pair, Lanczos, split, write back, update environments.
-    Returns (energy, discarded weight)."""
+class LanczosRequest(NamedTuple):
+    """One bond's Krylov step, handed to whoever solves it: the compiled operator, the FP64
+    refinement operator (or None), the start vector and the Krylov dimension."""
+
+    op: Any
+    refine: Any
+    aa: SymmetricTensor
+    ncv: int
+
+
+def _bond_gen(psi, env, schmidt, n, forward, *, chi, cutoff, ncv, noise, seed, precision,
+              refine, orthogonal_to=()):
+    """One bond of a two-site sweep as a generator: pair, hand the Lanczos step out
+    (``yield LanczosRequest``; the caller sends back ``(energy, ground vector)``), split,
+    write back, update environments.  Returns (energy, discarded weight)."""
     aa = _pair(psi[n], psi[n + 1])
     op = env.operator(n, precision)
     ref = env.operator(n, "fp64") if (precision != "fp64" and refine) else None
-    energy, aa = lanczos(op, aa, ncv=ncv, refine=ref)
+    energy, aa = yield LanczosRequest(op, ref, aa, ncv)
     if noise:
         r = SymmetricTensor.random(aa.legs, seed=seed + n, device=aa.device)
         aa = ops.add(aa, ops.multiply(r, float(noise * ops.norm(aa) / ops.norm(r))))
@@ -196,23 +208,39 @@ def _bond_step(psi, env, schmidt, n, forward, *, chi, cutoff, ncv, noise, seed, 
     return energy, dw
 
 
+def _bond_step(psi, env, schmidt, n, forward, **kw):
+    """``_bond_gen`` with its Lanczos step solved here, one problem at a time."""
+    g = _bond_gen(psi, env, schmidt, n, forward, **kw)
+    req = next(g)
+    try:
+        g.send(lanczos(req.op, req.aa, ncv=req.ncv, refine=req.refine))
+    except StopIteration as stop:
+        return stop.value
+    raise RuntimeError("a bond step asked for more than one Lanczos solve")
+
+
 def sweep_steps(psi, h, env, schmidt: dict, *, chi: int, cutoff: float, ncv: int = 3,
                 noise: float = 0.0, noise_type: str = "wavefunction", orthogonal_to=(),
-                seed: int = 0, precision: str = "fp64", refine: bool = True):
+                seed: int = 0, precision: str = "fp64", refine: bool = True, external: bool = False):
     """``sweep_`` as a generator that yields ``(energy, discarded)`` after every bond, so a
-    scheduler can interleave bonds of independent solves."""
+    scheduler can interleave bonds of independent solves.  With ``external`` it also yields
+    each bond's ``LanczosRequest`` first and expects ``(energy, vector)`` sent back, so a
+    scheduler can solve the Krylov steps of many solves together."""
     if noise_type not in ("wavefunction", "perturbative"):
         raise ValueError(f"noise_type={noise_type!r} is not one of ('wavefunction', 'perturbative')")
     if noise and noise_type == "perturbative":
         raise NotImplementedError("perturbative noise is not part of this build")
     n_sites = len(psi)
+    kw = dict(chi=chi, cutoff=cutoff, ncv=ncv, noise=noise, seed=seed, precision=precision,
+              refine=refine, orthogonal_to=orthogonal_to)
     for direction in ("right", "left"):
         forward = direction == "right"
         bonds = range(n_sites - 1) if forward else range(n_sites - 2, -1, -1)
         for n in bonds:
-            yield _bond_step(psi, env, schmidt, n, forward, chi=chi, cutoff=cutoff, ncv=ncv,
-                             noise=noise, seed=seed, precision=precision, refine=refine,
-                             orthogonal_to=orthogonal_to)
+            if external:
+                yield (yield from _bond_gen(psi, env, schmidt, n, forward, **kw))
+            else:
+                yield _bond_step(psi, env, schmidt, n, forward, **kw)
 
 
 def sweep_(psi, h, env, schmidt: dict, *, chi: int, cutoff: float, ncv: int = 3,
@@ -248,9 +276,11 @@ def _schmidt_change(old: dict, new: dict) -> float:
 def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1e-12,
                schmidt_tol: float = 1e-8, max_sweeps: int = 40, ncv: int = 3, seed: int = 0,
                callback=None, compile=None, precision: str = "fp64", refine: bool = True,
-               polish_tol: float | None = None):
+               polish_tol: float | None = None, external: bool = False):
     """``dmrg_`` as a generator: yields ``None`` after every bond and returns the
-    ``DMRG_out`` (``StopIteration.value``) when the loop ends.
+    ``DMRG_out`` (``StopIteration.value``) when the loop ends.  With ``external`` every
+    yield is instead one bond's ``LanczosRequest`` and the caller sends back
+    ``(energy, ground vector)``: the Krylov steps of many solves can then run as one batch.
 
     With a reduced ``precision`` the Schmidt values carry the operator's rounding
     (|dSchmidt| ~ 1e-6 for FP32), so the Schmidt criterion cannot be met by reduced
@@ -278,11 +308,24 @@ def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: floa
         entry = plan[it - 1] if it <= len(plan) else plan[-1]
         old_e, old_s = energy, dict(schmidt)
         max_dw = 0.0
-        for e, dw in sweep_steps(psi, h, env, schmidt, chi=entry.chi, cutoff=entry.cutoff, ncv=ncv,
-                                 noise=entry.noise, noise_type=entry.noise_type,
-                                 seed=seed + 977 * it, precision=precision, refine=refine):
+        steps = sweep_steps(psi, h, env, schmidt, chi=entry.chi, cutoff=entry.cutoff, ncv=ncv,
+                            noise=entry.noise, noise_type=entry.noise_type,
+                            seed=seed + 977 * it, precision=precision, refine=refine,
+                            external=external)
+        answer = None
+        while True:
+            try:
+                item = steps.send(answer)
+            except StopIteration:
+                break
+            answer = None
+            if isinstance(item, LanczosRequest):
+                answer = yield item
+                continue
+            e, dw = item
             energy, max_dw = e, max(max_dw, dw)
-            yield None
+            if not external:
+                yield None
         de = abs(old_e - energy)
         if precision != "fp64" and de < polish_tol:
             precision = "fp64"

@@ -82,16 +82,28 @@ def footprint_bytes(req: SolveRequest, itemsize: int = 8) -> int:
 class _Active:
     req: SolveRequest
     gen: object
-    stream: torch.cuda.Stream
+    stream: torch.cuda.Stream | None
     admitted: float
     bytes: int
+    pending: object = None  # shared mode: the solve's next LanczosRequest
 
 
 class DMRGServer:
-    """Admission by memory footprint, one stream per active solve, bond-level interleaving."""
+    """Admission by memory footprint, bond-level interleaving of the active solves.
+
+    ``mode="streams"``: every active solve runs its bond steps (its own Lanczos included)
+    on its own HIP stream, round robin.  ``mode="shared"``: on every scheduler tick the
+    pending Lanczos steps of ALL active solves run as one batch -- one ``BatchedHeff2``
+    matvec per Krylov iteration for every solve together, segmented Krylov scalars, one
+    batched tridiagonal solve and the batched FP64 refinement (``BatchedLanczos``) -- and
+    then each solve finishes its bond (split, environments) and hands in its next Lanczos
+    step.  A solve's Krylov arithmetic per problem is the same in any mix except the K
+    slicing of the batched GEMMs, so its result agrees alone and in the mix to rounding."""
 
     def __init__(self, *, memory_budget: int | None = None, max_active: int = 8,
-                 precision: str = "fp32", device="cuda") -> None:
+                 precision: str = "fp32", device="cuda", mode: str = "streams") -> None:
+        if mode not in ("streams", "shared"):
+            raise ValueError(f"unknown serving mode {mode!r}")
         if memory_budget is None:
             free, _ = torch.cuda.mem_get_info()
             memory_budget = int(0.8 * free)
@@ -99,7 +111,9 @@ class DMRGServer:
         self.max_active = max_active
         self.precision = precision
         self.device = device
+        self.mode = mode
         self.streams = [torch.cuda.Stream(device=device) for _ in range(max_active)]
+        self.ticks = 0  # batched Lanczos rounds run by the shared mode
 
     def _start(self, req: SolveRequest, stream, now: float) -> _Active:
         with torch.cuda.stream(stream):
@@ -122,9 +136,72 @@ class DMRGServer:
                 return SolveResult(a.req.rid, out.energy, out.sweeps, list(out.history),
                                    a.req.arrival, a.admitted, clock())
 
+    def _solve_batch(self, reqs) -> list:
+        """One ``BatchedLanczos`` per (operator precision, refinement) group of the pending
+        Lanczos requests; returns ``(energy, vector)`` per request, in order."""
+        from tenet.network.batched import BatchedHeff2
+        from tenet.network.batched_lanczos import BatchedLanczos
+
+        groups: dict = {}
+        for k, r in enumerate(reqs):
+            groups.setdefault((r.op.precision, r.refine is not None, r.ncv), []).append(k)
+        answers = [None] * len(reqs)
+        for (prec, has_ref, ncv), ks in groups.items():
+            batch = BatchedHeff2(None, prec, operators=[(reqs[k].op, reqs[k].aa.structure) for k in ks])
+            ref = (BatchedHeff2(None, "fp64", operators=[(reqs[k].refine, reqs[k].aa.structure) for k in ks])
+                   if has_ref else None)
+            vals, vecs = BatchedLanczos(batch, ncv=ncv, refine=ref, graphs=False).solve([reqs[k].aa for k in ks])
+            for k, e, v in zip(ks, vals, vecs):
+                answers[k] = (e, v)
+        self.ticks += 1
+        return answers
+
+    def _start_shared(self, req: SolveRequest, now: float) -> _Active:
+        psi = random_chain(req.n_sites, max(8, req.chi // 4), 3.0, seed=req.seed, device=self.device)
+        h = heisenberg_mpo(req.n_sites, req.delta, device=self.device)
+        gen = dmrg_steps(psi, h, chi=req.chi, cutoff=req.cutoff, energy_tol=req.energy_tol,
+                         schmidt_tol=req.schmidt_tol, max_sweeps=req.max_sweeps,
+                         precision=self.precision, external=True)
+        a = _Active(req, gen, None, now, footprint_bytes(req))
+        a.pending = next(gen)
+        return a
+
+    def _serve_shared(self, requests, clock, on_result) -> list:
+        queue = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        t0 = clock()
+        active, done, used, qi = [], [], 0, 0
+        while qi < len(queue) or active:
+            now = clock() - t0
+            while (qi < len(queue) and queue[qi].arrival <= now and len(active) < self.max_active
+                   and (not active or used + footprint_bytes(queue[qi]) <= self.memory_budget)):
+                a = self._start_shared(queue[qi], now)
+                qi += 1
+                used += a.bytes
+                active.append(a)
+            if not active:
+                time.sleep(max(0.0, min(0.01, queue[qi].arrival - now)))
+                continue
+            answers = self._solve_batch([a.pending for a in active])
+            for a, ans in zip(list(active), answers):
+                try:
+                    a.pending = a.gen.send(ans)
+                except StopIteration as stop:
+                    out = stop.value
+                    torch.cuda.synchronize()
+                    res = SolveResult(a.req.rid, out.energy, out.sweeps, list(out.history),
+                                      a.req.arrival, a.admitted, clock() - t0)
+                    done.append(res)
+                    if on_result is not None:
+                        on_result(res)
+                    active.remove(a)
+                    used -= a.bytes
+        return sorted(done, key=lambda r: r.rid)
+
     def serve(self, requests, *, clock=time.perf_counter, on_result=None) -> list:
         """Run a request stream to completion; arrival times are relative to the start.
         ``on_result`` is called with each ``SolveResult`` as its solve converges."""
+        if self.mode == "shared":
+            return self._serve_shared(requests, clock, on_result)
         queue = sorted(requests, key=lambda r: (r.arrival, r.rid))
         t0 = clock()
         active: list = []

@@ -53,3 +53,23 @@ def test_multiprocess_server_matches_the_in_process_server():
         assert g.sweeps == r.sweeps
         assert abs(g.energy - r.energy) <= 1e-12 * abs(r.energy)
         assert g.admitted >= g.arrival - 1e-3 and g.finished > g.admitted
+
+
+def test_shared_batches_agree_alone_and_in_the_mix():
+    """The shared mode runs the pending Lanczos steps of every active solve as one batched
+    Krylov solve per tick; each solve's energy agrees alone and in the mix (the batched
+    GEMMs may slice K differently, so agreement is to rounding, not bitwise) and matches
+    exact diagonalization."""
+    from tenet.network.serve import DMRGServer
+
+    srv = DMRGServer(max_active=3, mode="shared")
+    mix = srv.serve(_reqs())
+    alone_ticks = 0
+    for r in _reqs():
+        solo = DMRGServer(max_active=1, mode="shared")
+        alone = solo.serve([r])[0]
+        alone_ticks += solo.ticks
+        got = mix[r.rid]
+        assert abs(got.energy - alone.energy) <= 1e-10 * abs(alone.energy)
+        assert abs(got.energy - ed.xxz_sz0_energy(r.n_sites, r.delta)) < 1e-8 * abs(got.energy)
+    assert srv.ticks < alone_ticks  # bonds of different solves did share batches
