@@ -178,7 +178,12 @@ int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t* jobs, int6
 #define TN_SVD_MAX_K 4096
 int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int64_t nwgs, int64_t span,
                         double* x, double* q, int32_t* rotations, int w, int round, int first,
-                        double tol, void* stream);
+                        double tol, const int32_t* stop, void* stream);
+/* *stop = 1 when all n rotation counts of a sweep (rotations of every round of one launch
+ * group) are zero.  With `stop` passed to tn_svd_jacobi_round (NULL: always run), the host
+ * queues several sweeps ahead and reads one flag instead of the counts of every sweep;
+ * rounds past convergence return at once. */
+int tn_svd_jacobi_converged(const int32_t* rotations, int64_t n, int32_t* stop, void* stream);
 int tn_svd_jacobi_finish(const int64_t* probs, int64_t nprobs, const double* x, const double* q,
                          double* U, double* S, double* Vh, void* stream);
 

@@ -62,9 +62,13 @@ __device__ inline void pair_of(int n, int r, int p, int& a, int& b) {
 __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* __restrict__ wgs,
                              double* __restrict__ x, double* __restrict__ q,
                              int32_t* __restrict__ rot_out, int w, int round, int first,
-                             double tol) {
+                             double tol, const int32_t* __restrict__ stop) {
   extern __shared__ double lds[];
   __shared__ int rot_s;
+  // the launch's problems have converged (set by jacobi_converged after an earlier sweep):
+  // the host queues sweeps ahead without reading the rotation counts back, and those past
+  // convergence leave every vector as it is
+  if (stop != nullptr && *stop != 0) return;
   const int64_t* pw = probs + TN_SVD_PROB_WORDS * wgs[2 * blockIdx.x];
   const int slot = wgs[2 * blockIdx.x + 1];
   const int k = static_cast<int>(pw[0]);
@@ -168,6 +172,20 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
     for (int i = threadIdx.x; i < k; i += blockDim.x) qr[i] = qs[t * k + i];
   }
   if (threadIdx.x == 0) rot_out[blockIdx.x] = last;
+}
+
+// *stop = 1 once a whole sweep of the launch group rotated nothing (n counts, one per
+// workgroup of every round of the sweep).
+__global__ __launch_bounds__(256) void jacobi_converged(const int32_t* __restrict__ rot, int64_t n,
+                                                       int32_t* __restrict__ stop) {
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) mine |= rot[i];
+  if (mine) atomicOr(&any, 1);
+  __syncthreads();
+  if (threadIdx.x == 0 && any == 0) *stop = 1;
 }
 
 // One workgroup per problem: singular values = row norms, descending order by rank
@@ -287,9 +305,17 @@ int allow_lds(const void* fn, size_t bytes, const char* who) {
 
 }  // namespace
 
+extern "C" int tn_svd_jacobi_converged(const int32_t* rotations, int64_t n, int32_t* stop, void* stream) {
+  if (n <= 0) return 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  jacobi_converged<<<1, 256, 0, s>>>(rotations, n, stop);
+  return tn::check_launch("jacobi_converged");
+}
+
 extern "C" int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int64_t nwgs,
                                    int64_t span, double* x, double* q, int32_t* rotations, int w,
-                                   int round, int first, double tol, void* stream) {
+                                   int round, int first, double tol, const int32_t* stop,
+                                   void* stream) {
   if (nwgs <= 0) return 0;
   if (w < 1 || w > TN_SVD_MAX_W) return tn::fail("tn_svd_jacobi_round: block width out of range");
   if (nwgs > 0x7fffffff) return tn::fail("tn_svd_jacobi_round: too many workgroups");
@@ -302,7 +328,7 @@ extern "C" int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int threads = w <= 2 ? 64 : 32 * w;
   hipLaunchKernelGGL(jacobi_round, dim3(static_cast<unsigned>(nwgs)), dim3(threads), bytes, s, probs,
-                     wgs, x, q, rotations, w, round, first, tol);
+                     wgs, x, q, rotations, w, round, first, tol, stop);
   return tn::check_launch("jacobi_round");
 }
 

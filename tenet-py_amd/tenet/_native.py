@@ -48,7 +48,8 @@ _SIGNATURES = {
     "tn_scale": (_i32, [_i32, _i64, _d, _p, _i32, _p, _p]),
     "tn_cast": (_i32, [_i32, _p, _i32, _p, _i64, _p]),
     "tn_mpo_apply": (_i32, [_i32, _p, _p, _p, _i64, _p, _p, _p, _p, _i64, _p]),
-    "tn_svd_jacobi_round": (_i32, [_p, _p, _i64, _i64, _p, _p, _p, _i32, _i32, _i32, _d, _p]),
+    "tn_svd_jacobi_round": (_i32, [_p, _p, _i64, _i64, _p, _p, _p, _i32, _i32, _i32, _d, _p, _p]),
+    "tn_svd_jacobi_converged": (_i32, [_p, _i64, _p, _p]),
     "tn_svd_jacobi_finish": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p]),
     "tn_tridiag_ground": (_i32, [_p, _p, _i32, _d, _p, _p, _p]),
     "tn_tridiag_ground_host": (_i32, [_p, _p, _i32, _d, _p, _p]),

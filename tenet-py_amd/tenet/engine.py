@@ -560,9 +560,12 @@ class GemmProgram:
         key = tile if grid <= 0 else (tile, grid)
         d = self._by_tile.get(key)
         if d is None and tile == "skinny":
-            lines = self.Ms if self.skinny else self.Ns
-            tiles = np.asarray([(p, b) for p, n in enumerate(lines) for b in range(-(-n // 256))],
-                               dtype=np.int32).reshape(-1, 2)
+            lines = np.asarray(self.Ms if self.skinny else self.Ns, dtype=np.int64)
+            per = -(-lines // 256)
+            total = int(per.sum())
+            tiles = np.empty((total, 2), dtype=np.int32)
+            tiles[:, 0] = np.repeat(np.arange(len(lines), dtype=np.int32), per)
+            tiles[:, 1] = np.arange(total, dtype=np.int64) - np.repeat(np.cumsum(per) - per, per)
             d = _DeviceCache(probs=self._probs.reshape(-1), segs=self._segs.reshape(-1),
                              tiles=tiles.reshape(-1))
             d.ntiles = len(tiles)

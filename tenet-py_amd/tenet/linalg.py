@@ -241,7 +241,8 @@ _SVD_LDS_BYTES = None     # per-workgroup LDS the block pair may use (device lim
 _SVD_MAX_SWEEPS = 60
 _SVD_INNER_SWEEPS = 1     # inner sweeps per block-pair visit when a problem spans several blocks
 _SVD_SINGLE_SWEEPS = 64   # cap for a problem that fits in one block pair (converges in-launch)
-_SVD_LAST_SWEEPS = 0      # sweeps the last call needed (diagnostics)
+_SVD_LAST_SWEEPS = 0      # sweeps queued by the last call (diagnostics)
+_SVD_SWEEP_BATCH = 8      # sweeps queued before the first read of the convergence flags
 
 
 def _lds_budget() -> int:
@@ -350,23 +351,29 @@ def _svd_parts(packed) -> dict:
     for g in plan.groups:
         g["tb"] = g["tables"].get(dev)
         rots.append(torch.zeros(max(g["rounds"], 1) * g["nwgs"], dtype=torch.int32, device=dev))
-    live = list(range(len(plan.groups)))
+    # one convergence flag per launch group, set on the device after a sweep that rotated
+    # nothing; sweeps are queued in batches and the host reads the flags once per batch
+    stop = torch.zeros(len(plan.groups), dtype=torch.int32, device=dev)
     global _SVD_LAST_SWEEPS
-    for sweep in range(_SVD_MAX_SWEEPS):
-        _SVD_LAST_SWEEPS = sweep + 1
-        for gi in live:
-            g, rot = plan.groups[gi], rots[gi]
-            for r in range(g["rounds"]):
-                _native.check(lib.tn_svd_jacobi_round(
-                    g["tb"]["probs"].data_ptr(), g["tb"]["wgs"].data_ptr(), g["nwgs"], g["span"],
-                    x.data_ptr(), q.data_ptr(), rot[r * g["nwgs"]:].data_ptr(), g["w"], r,
-                    int(sweep == 0 and r == 0), g["tol"], stream))
-        counts = torch.stack([rots[gi].sum() for gi in live]).cpu().tolist()
-        live = [gi for gi, n in zip(live, counts) if n]
-        if not live:
+    done = 0
+    while True:
+        batch = min(_SVD_SWEEP_BATCH if done == 0 else _SVD_SWEEP_BATCH // 2, _SVD_MAX_SWEEPS - done)
+        for sweep in range(done, done + batch):
+            for gi, (g, rot) in enumerate(zip(plan.groups, rots)):
+                flag = stop[gi:]
+                for r in range(g["rounds"]):
+                    _native.check(lib.tn_svd_jacobi_round(
+                        g["tb"]["probs"].data_ptr(), g["tb"]["wgs"].data_ptr(), g["nwgs"], g["span"],
+                        x.data_ptr(), q.data_ptr(), rot[r * g["nwgs"]:].data_ptr(), g["w"], r,
+                        int(sweep == 0 and r == 0), g["tol"], flag.data_ptr(), stream))
+                _native.check(lib.tn_svd_jacobi_converged(rot.data_ptr(), g["rounds"] * g["nwgs"],
+                                                          flag.data_ptr(), stream))
+        done += batch
+        if bool(stop.all()):
             break
-    else:
-        raise RuntimeError(f"svd: the Jacobi sweeps did not converge in {_SVD_MAX_SWEEPS} sweeps")
+        if done >= _SVD_MAX_SWEEPS:
+            raise RuntimeError(f"svd: the Jacobi sweeps did not converge in {_SVD_MAX_SWEEPS} sweeps")
+    _SVD_LAST_SWEEPS = done
     U = torch.empty(plan.u_size, dtype=torch.float64, device=dev)
     S = torch.empty(plan.s_size, dtype=torch.float64, device=dev)
     Vh = torch.empty(plan.v_size, dtype=torch.float64, device=dev)

@@ -19,6 +19,7 @@ splits; perturbative noise is not part of this build).
 
 from __future__ import annotations
 
+import gc
 from collections.abc import Callable, Sequence
 from typing import Any, NamedTuple
 
@@ -53,8 +54,9 @@ class DMRG_out(NamedTuple):
 def lanczos(matvec: Callable[[SymmetricTensor], SymmetricTensor], v: SymmetricTensor, *,
             ncv: int = 3, tol: float = 1e-13, orthogonal_to: Sequence[SymmetricTensor] = (),
             refine: Callable[[SymmetricTensor], SymmetricTensor] | None = None,
-            reorthogonalize: bool = True) -> tuple[float, SymmetricTensor]:
-    """Ground eigenpair ``(value, vector)`` of a Hermitian ``matvec``."""
+            reorthogonalize: bool = True, host_value: bool = True) -> tuple[float, SymmetricTensor]:
+    """Ground eigenpair ``(value, vector)`` of a Hermitian ``matvec``.  ``host_value=False``
+    returns the value as a 0-d device tensor (no host synchronisation)."""
     basis = []
     for t in orthogonal_to:
         for b in basis:
@@ -99,7 +101,7 @@ def lanczos(matvec: Callable[[SymmetricTensor], SymmetricTensor], v: SymmetricTe
     out = ops.divide(out, ops.norm(out))
     if refine is not None:
         value, out, _ = residual_refine(refine, out)
-    return float(value), out
+    return (float(value) if host_value else value), out
 
 
 def ritz(alphas: torch.Tensor, betas: torch.Tensor, basis, tol: float, coef=None, value=None, out=None):
@@ -152,11 +154,18 @@ def residual_refine(op64: Callable[[SymmetricTensor], SymmetricTensor], psi: Sym
 
 def spectrum(s: SymmetricTensor) -> list:
     """Every diagonal value of a singular-value tensor, ``sqrt(qdim)``-weighted, descending."""
+    return _spectrum_of(s.structure, s.data.detach().to("cpu", torch.float64).numpy())
+
+
+def _spectrum_of(structure, flat: np.ndarray) -> list:
+    """``spectrum`` of a diagonal tensor of ``structure`` whose flat blocks are on the host."""
+    prov = structure.provider
+    lay = structure.layout
     vals = []
-    prov = s.provider
-    for key, blk in zip(s.structure.block_order, s.numpy_blocks()):
-        w = prov.qdim(key.coupled) ** 0.5
-        vals.extend((np.diag(blk) * w).tolist())
+    for key, off, shp in zip(structure.block_order, lay.offsets.tolist(), lay.shapes.tolist()):
+        k = shp[0]
+        diag = flat[off: off + k * k: k + 1]
+        vals.extend((diag * prov.qdim(key.coupled) ** 0.5).tolist())
     return sorted(vals, reverse=True)
 
 
@@ -188,8 +197,10 @@ def _bond_gen(psi, env, schmidt, n, forward, *, chi, cutoff, ncv, noise, seed, p
         aa = ops.add(aa, ops.multiply(r, float(noise * ops.norm(aa) / ops.norm(r))))
         aa = ops.divide(aa, ops.norm(aa))
     u, s, vh = linalg.svd_truncated(aa, ((0, 1), (2, 3)), max_bond=chi, cutoff=cutoff)
-    norm_s = float(ops.norm(s))
-    dw = 1.0 - (norm_s / float(ops.norm(aa))) ** 2
+    norm_s = ops.norm(s)
+    # one readback per bond for the norms and the Schmidt values
+    host = torch.cat([norm_s.view(1), ops.norm(aa).view(1), s.data.to(torch.float64)]).cpu().numpy()
+    dw = 1.0 - (host[0] / host[1]) ** 2
     s = ops.divide(s, norm_s)
     psi[n + 1] = vh
     if forward:
@@ -197,7 +208,7 @@ def _bond_gen(psi, env, schmidt, n, forward, *, chi, cutoff, ncv, noise, seed, p
         psi[n + 1] = ops.einsum("xy,yqr->xqr", s, psi[n + 1])
     else:
         psi[n] = ops.einsum("apx,xy->apy", u, s)
-    schmidt[n] = spectrum(s)
+    schmidt[n] = _spectrum_of(s.structure, host[2:] / host[0])
     psi.center = n + 1 if forward else n
     for e in (env, *orthogonal_to):
         e.clear_(n, n + 1)
@@ -213,7 +224,7 @@ def _bond_step(psi, env, schmidt, n, forward, **kw):
     g = _bond_gen(psi, env, schmidt, n, forward, **kw)
     req = next(g)
     try:
-        g.send(lanczos(req.op, req.aa, ncv=req.ncv, refine=req.refine))
+        g.send(lanczos(req.op, req.aa, ncv=req.ncv, refine=req.refine, host_value=False))
     except StopIteration as stop:
         return stop.value
     raise RuntimeError("a bond step asked for more than one Lanczos solve")
@@ -257,7 +268,7 @@ def sweep_(psi, h, env, schmidt: dict, *, chi: int, cutoff: float, ncv: int = 3,
                              noise_type=noise_type, orthogonal_to=orthogonal_to, seed=seed,
                              precision=precision, refine=refine):
         energy, max_dw = e, max(max_dw, dw)
-    return energy, max_dw
+    return float(energy), max_dw
 
 
 def _schmidt_change(old: dict, new: dict) -> float:
@@ -276,11 +287,17 @@ def _schmidt_change(old: dict, new: dict) -> float:
 def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1e-12,
                schmidt_tol: float = 1e-8, max_sweeps: int = 40, ncv: int = 3, seed: int = 0,
                callback=None, compile=None, precision: str = "fp64", refine: bool = True,
-               polish_tol: float | None = None, external: bool = False):
+               polish_tol: float | None = None, external: bool = False, gc_freeze: bool = True):
     """``dmrg_`` as a generator: yields ``None`` after every bond and returns the
     ``DMRG_out`` (``StopIteration.value``) when the loop ends.  With ``external`` every
     yield is instead one bond's ``LanczosRequest`` and the caller sends back
     ``(energy, ground vector)``: the Krylov steps of many solves can then run as one batch.
+
+    ``gc_freeze``: after every sweep the objects alive at that point (compiled plans and
+    device tables, mostly long-lived) are moved out of the cyclic collector's generations
+    (``gc.freeze``), so its full collections do not re-traverse the plan caches on every
+    bond (they cost ~15% of a steady sweep otherwise); reference counting still frees
+    everything that dies.
 
     With a reduced ``precision`` the Schmidt values carry the operator's rounding
     (|dSchmidt| ~ 1e-6 for FP32), so the Schmidt criterion cannot be met by reduced
@@ -326,6 +343,9 @@ def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: floa
             energy, max_dw = e, max(max_dw, dw)
             if not external:
                 yield None
+        energy = float(energy)  # the sweep's last bond energy (a device scalar until here)
+        if gc_freeze:
+            gc.freeze()
         de = abs(old_e - energy)
         if precision != "fp64" and de < polish_tol:
             precision = "fp64"
