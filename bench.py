@@ -500,6 +500,43 @@ def bench_batched(args) -> int:
                       "fixed_tile_ms": dt_fixed * 1e3, "tuned_tiles": {k: list(v) for k, v in tuned.items()},
                       "speedup_tuned_vs_fixed": dt_fixed / dt_tuned, "flops": bh.flops}
         log(f"{p}: fixed {dt_fixed * 1e3:.3f} ms, tuned {dt_tuned * 1e3:.3f} ms ({tuned})")
+        # the Lanczos step of every problem at once: ncv graph-replayed iterations (one
+        # batched matvec each, segmented FP64 scalars) and the batched tridiagonal finish;
+        # the reduced-precision arms close with the batched FP64 residual refinement
+        from tenet.network.batched_lanczos import BatchedLanczos
+
+        start = torch.cat([i[4].data for i in insts])
+        bl = BatchedLanczos(bh, ncv=3)
+        bl.start_(start)
+        dt_l = time_arm(bl.run, max(2, args.steps // 4), 2)
+        results[p]["lanczos"] = {"ncv": 3, "ms_per_batched_step": dt_l * 1e3,
+                                 "problem_lanczos_steps_per_s": len(insts) / dt_l,
+                                 "launch": "one hipGraph per iteration and one for the finish"}
+        log(f"{p}: batched Lanczos step (ncv 3) {dt_l * 1e3:.3f} ms for {len(insts)} problems")
+        if p != "fp64":
+            blr = BatchedLanczos(bh, ncv=3, refine=BatchedHeff2(insts, "fp64"))
+            blr.start_(start)
+            dt_r = time_arm(blr.run, max(2, args.steps // 4), 2)
+            results[p]["lanczos"]["ms_per_batched_step_with_fp64_refine"] = dt_r * 1e3
+            log(f"{p}: with the batched FP64 refinement {dt_r * 1e3:.3f} ms")
+        if p == "fp32":
+            # one problem at a time: the single-problem graph-captured Krylov step, a sample
+            from tenet.network import Heff2
+            from tenet.network.graph import KrylovGraph
+
+            sample = insts[:: max(1, len(insts) // 16)]
+            kgs = [(KrylovGraph(Heff2(*i[:4], "fp32"), i[4].structure, ncv=3), i[4]) for i in sample]
+            for kg, v in kgs:
+                kg.start_(v.data)
+
+            def serial():
+                for kg, _ in kgs:
+                    kg.replay()
+
+            dt_s = time_arm(serial, max(2, args.steps // 4), 2) * len(insts) / len(sample)
+            results[p]["lanczos"]["one_at_a_time_ms"] = dt_s * 1e3
+            results[p]["lanczos"]["speedup_batched_vs_one_at_a_time"] = dt_s / dt_l
+            log(f"fp32: one problem at a time {dt_s * 1e3:.3f} ms for {len(insts)} problems")
         if p == "fp32":
             # the same batch with one GEMM problem per block pair (no sector stacking)
             bb = BatchedHeff2(insts, p, layout="blocks")
