@@ -8,6 +8,7 @@ from tenet.dense import from_dense, to_dense
 from tenet.engine import gemm_precision, set_gemm_precision
 from tenet.fusion_tree import FusionTree, coupled_sectors, fusion_trees
 from tenet.ops import (
+    block_sqrt,
     add,
     adjoint,
     allclose,
@@ -47,6 +48,7 @@ __all__ = [
     "adjoint",
     "allclose",
     "astype",
+    "block_sqrt",
     "compose",
     "conj",
     "coupled_sectors",

@@ -13,8 +13,9 @@ takes an FP64 matvec and closes the step with ``residual_refine``: a Rayleigh-Ri
 correction in the span of the Ritz vector and its FP64 residual, which removes the FP32
 operator's error from the energy and the vector.
 
-``sweep_``/``dmrg_`` follow reference ``dmrg.py:321-843`` (noiseless and wavefunction-noise
-splits; perturbative noise is not part of this build).
+``sweep_``/``dmrg_`` follow reference ``dmrg.py:321-843``: the noiseless SVD split, the
+wavefunction-noise split, and perturbative noise (the density-matrix decimation with the
+operator's own action on the two-site tensor folded into ``rho``).
 """
 
 from __future__ import annotations
@@ -183,8 +184,45 @@ class LanczosRequest(NamedTuple):
     ncv: int
 
 
+def _rho(m: SymmetricTensor, forward: bool) -> SymmetricTensor:
+    """``m m^dag`` going forward, ``m^dag m`` coming back, for ``m`` on ``(l, p | q, r)``
+    (reference ``network/dmrg.py:245-253``)."""
+    return ops.compose(m, ops.adjoint(m)) if forward else ops.compose(ops.adjoint(m), m)
+
+
+def _perturbations(env, n: int, aa: SymmetricTensor, noise: float) -> tuple:
+    """The perturbation vectors of perturbative noise (reference ``dmrg.py:256-278``): the
+    operator's term families on ``aa``, each normalised, the set scaled to total squared
+    norm ``noise``; a numerically zero family is dropped."""
+    parts = [(p, float(ops.norm(p))) for p in env.heff2_families(n, aa)]
+    live = [(p, w) for p, w in parts if w > 0.0]
+    if not live:
+        return ()
+    scale = (noise / len(live)) ** 0.5
+    return tuple(ops.multiply(p, scale / w) for p, w in live)
+
+
+def _split_dm(aa: SymmetricTensor, forward: bool, *, chi: int, cutoff: float, perturbations=()):
+    """The density-matrix decimation (reference ``dmrg.py:281-322``): ``svd_truncated`` of
+    ``rho = m m^dag`` (forward) or ``m^dag m`` with every perturbation's ``p p^dag`` added,
+    in cutoff mode ``rsum1`` (on rho's spectrum, the same rule as ``rsum2`` on ``aa``'s).
+    Returns ``(left, right, s)``: the isometry on the side the sweep leaves, the weighted
+    factor on the other, and the singular values ``sqrt(spectrum)``."""
+    m = ops.repartition(aa, (0, 1), (2, 3))
+    rho = _rho(m, forward)
+    for p in perturbations:
+        rho = ops.add(rho, _rho(ops.repartition(p, (0, 1), (2, 3)), forward))
+    v, w, _ = linalg.svd_truncated(rho, max_bond=chi, cutoff=cutoff, cutoff_mode="rsum1")
+    s = ops.block_sqrt(w)
+    if forward:
+        return v, ops.compose(ops.adjoint(v), m), s
+    # the adjoint keeps v's axis order, so the new bond comes last; the next site wants it
+    # first
+    return ops.compose(m, v), ops.transpose(ops.adjoint(v), (2, 0, 1)), s
+
+
 def _bond_gen(psi, env, schmidt, n, forward, *, chi, cutoff, ncv, noise, seed, precision,
-              refine, orthogonal_to=()):
+              refine, orthogonal_to=(), noise_type: str = "wavefunction"):
     """One bond of a two-site sweep as a generator: pair, hand the Lanczos step out
     (``yield LanczosRequest``; the caller sends back ``(energy, ground vector)``), split,
     write back, update environments.  Returns (energy, discarded weight)."""
@@ -192,6 +230,26 @@ def _bond_gen(psi, env, schmidt, n, forward, *, chi, cutoff, ncv, noise, seed, p
     op = env.operator(n, precision)
     ref = env.operator(n, "fp64") if (precision != "fp64" and refine) else None
     energy, aa = yield LanczosRequest(op, ref, aa, ncv)
+    if noise and noise_type == "perturbative":
+        left, right, s = _split_dm(aa, forward, chi=chi, cutoff=cutoff,
+                                   perturbations=_perturbations(env, n, aa, noise))
+        # the factor that carries the weight is the truncated state, the other an
+        # isometry: its norm is the kept fraction (the SVD split's Pythagoras)
+        carrier = right if forward else left
+        norm_c = ops.norm(carrier)
+        host = torch.cat([norm_c.view(1), ops.norm(aa).view(1)]).cpu().numpy()
+        dw = 1.0 - (host[0] / host[1]) ** 2
+        psi[n] = left if forward else ops.divide(left, norm_c)
+        psi[n + 1] = ops.divide(right, norm_c) if forward else right
+        schmidt[n] = spectrum(ops.divide(s, ops.norm(s)))
+        psi.center = n + 1 if forward else n
+        for e in (env, *orthogonal_to):
+            e.clear_(n, n + 1)
+            if forward:
+                e.update_(n, to="last")
+            else:
+                e.update_(n + 1, to="first")
+        return energy, dw
     if noise:
         r = SymmetricTensor.random(aa.legs, seed=seed + n, device=aa.device)
         aa = ops.add(aa, ops.multiply(r, float(noise * ops.norm(aa) / ops.norm(r))))
@@ -239,11 +297,9 @@ def sweep_steps(psi, h, env, schmidt: dict, *, chi: int, cutoff: float, ncv: int
     scheduler can solve the Krylov steps of many solves together."""
     if noise_type not in ("wavefunction", "perturbative"):
         raise ValueError(f"noise_type={noise_type!r} is not one of ('wavefunction', 'perturbative')")
-    if noise and noise_type == "perturbative":
-        raise NotImplementedError("perturbative noise is not part of this build")
     n_sites = len(psi)
     kw = dict(chi=chi, cutoff=cutoff, ncv=ncv, noise=noise, seed=seed, precision=precision,
-              refine=refine, orthogonal_to=orthogonal_to)
+              refine=refine, orthogonal_to=orthogonal_to, noise_type=noise_type)
     for direction in ("right", "left"):
         forward = direction == "right"
         bonds = range(n_sites - 1) if forward else range(n_sites - 2, -1, -1)

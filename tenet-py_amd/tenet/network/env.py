@@ -110,6 +110,13 @@ class Env:
         """``H_eff @ aa`` on bond ``(n, n+1)``; the result has ``aa``'s structure."""
         return heff2_full(self.h, self.F[n - 1, n], self.F[n + 2, n + 1], n, aa)
 
+    def heff2_families(self, n: int, aa: SymmetricTensor) -> tuple:
+        """``heff2``'s term families applied separately (reference ``env.py:772-841``).  The
+        two-site H_eff here always runs on the MPO's site tensors, where the operator has
+        no families to resolve: the single vector ``(heff2(n, aa),)``, as the reference's
+        site-tensor path returns."""
+        return (self.heff2(n, aa),)
+
     def operator(self, n: int, precision: str = "fp64"):
         """The compiled H_eff of bond ``(n, n+1)`` at ``precision`` (cached until the
         bond's environments are cleared)."""

@@ -29,7 +29,7 @@ from tenet.symmetry import SU2, U1, SU2Sector, U1Sector
 
 __all__ = ["PHYS", "BOUNDARY", "MPO_BOND", "spin_half_w", "heisenberg_mpo", "gaussian_bond",
            "chain_bonds", "random_chain", "SU2_PHYS", "SU2_BOUNDARY", "SU2_MPO_BOND",
-           "su2_heisenberg_w", "su2_heisenberg_mpo", "su2_gaussian_bond", "su2_chain_bonds",
+           "su2_heisenberg_w", "su2_heisenberg_mpo", "su2_heisenberg_mpo_w", "su2_gaussian_bond", "su2_chain_bonds",
            "su2_random_chain"]
 
 PHYS = GradedSpace.new(U1, {U1Sector(-1): 1, U1Sector(1): 1})
@@ -156,6 +156,18 @@ def su2_heisenberg_w(j: float = 1.0) -> np.ndarray:
 
 
 def su2_heisenberg_mpo(n_sites: int, j: float = 1.0, device=None) -> MPO:
+    """``H = J sum_i S_i . S_{i+1}`` as one invariant two-site term per bond through
+    ``MPO.from_terms`` and ``materialize`` (reference ``examples/su2_heisenberg.py:27-31``):
+    the MPO bond is the two corner singlets plus the spin-1 channel of ``S . S``."""
+    from tenet.network.terms import spin_half
+
+    ss = spin_half(SU2, device=device).ops["S.S"]
+    return MPO.from_terms(n_sites, [(j, [(ss, (i, i + 1))]) for i in range(n_sites - 1)],
+                          device=device).materialize()
+
+
+def su2_heisenberg_mpo_w(n_sites: int, j: float = 1.0, device=None) -> MPO:
+    """The same operator from the hand-written invariant ``W`` of ``su2_heisenberg_w``."""
     return MPO.from_w(su2_heisenberg_w(j), n_sites, phys=SU2_PHYS, bond=SU2_MPO_BOND,
                       boundary=SU2_BOUNDARY, start=SU2_START, end=SU2_END, device=device)
 

@@ -64,6 +64,55 @@ class MPS:
             for i in range(len(bonds) - 1)
         )
 
+    @classmethod
+    def product(cls, phys: GradedSpace, states: Sequence, *, dtype=None, device=None) -> "MPS":
+        """A norm-1 product state, site ``n`` in the basis vector of sector ``states[n]``
+        (reference ``mps.py:193-290``).  The bonds follow from the charges, right to left:
+        the last bond is the unit sector and bond ``n`` is bond ``n + 1`` fused with the
+        dual of ``states[n]``, so bond 0 carries the total charge; its leg shows the total
+        itself and is dual when that is not the unit.  Refuses a sector absent from
+        ``phys`` or of degeneracy above 1, and a fusion with more than one channel."""
+        prov = phys.provider
+        states = list(states)
+        for n, a in enumerate(states):
+            if phys.degeneracy(a) != 1:
+                raise ValueError(f"MPS.product: sector {a!r} at site {n} must appear in phys with "
+                                 f"degeneracy 1, it has {phys.degeneracy(a)}")
+        bonds = [prov.unit]
+        for n in range(len(states) - 1, -1, -1):
+            out = prov.fusion(bonds[0], prov.dual(states[n]))
+            if len(out) != 1:
+                raise ValueError(f"MPS.product: bond {n + 1} x dual({states[n]!r}) has {len(out)} "
+                                 "fusion channels; a product state needs one")
+            bonds.insert(0, out[0])
+        total = prov.dual(bonds[0])
+        first = Leg(GradedSpace.new(prov, {total: 1}), OUT, total != prov.unit)
+        offsets, pos = {}, 0
+        for a, m in phys.sectors:
+            offsets[a] = pos
+            pos += int(m * prov.qdim(a))
+        sites = []
+        for n, a in enumerate(states):
+            arr = np.zeros((1, phys.dim, 1))
+            arr[0, offsets[a], 0] = 1.0
+            left = first if n == 0 else Leg(GradedSpace.new(prov, {bonds[n]: 1}), OUT)
+            legs = (left, Leg(phys, OUT), Leg(GradedSpace.new(prov, {bonds[n + 1]: 1}), IN))
+            t = from_dense(arr, legs, device=device)
+            sites.append(t if dtype is None else ops.astype(t, dtype))
+        return cls(sites)
+
+    def norm(self) -> float:
+        """``sqrt(<psi|psi>)``: an LQ pass from the right moves the whole state's weight
+        into site 0 through right isometries, whose norm is then the state's (the state
+        itself is not changed)."""
+        if not self.sites:
+            return 0.0
+        head = self.sites[-1]
+        for n in range(len(self) - 1, 0, -1):
+            left, _ = linalg.lq(head, ((0,), (1, 2)))
+            head = ops.einsum("apx,xy->apy", self.sites[n - 1], left)
+        return float(ops.norm(head))
+
     def canonize_(self, to: int = 0) -> "MPS":
         """Right-canonical form by an LQ sweep from the right (reference ``mps.py:312-352``)."""
         if to != 0:
@@ -87,11 +136,16 @@ def from_dense(array, legs: Sequence[Leg], *, atol: float = 1e-10, device=None) 
 
 
 class MPO:
-    """A finite MPO of site tensors ``(wl IN, p OUT, p IN, wr OUT)``."""
+    """A finite MPO of site tensors ``(wl IN, p OUT, p IN, wr OUT)``.
 
-    def __init__(self, sites: Iterable[SymmetricTensor]) -> None:
+    ``edges`` is the channel table of an MPO built by ``from_terms`` (which open term
+    occupies which graded channel at which cut) and ``None`` otherwise; ``materialize``
+    returns the same site tensors without it (reference ``network/mps.py:2551-2690``).
+    The two-site H_eff always runs on the site tensors."""
+
+    def __init__(self, sites: Iterable[SymmetricTensor], *, edges=None) -> None:
         self.sites = list(sites)
-        self.edges = None
+        self.edges = edges
 
     def __len__(self) -> int:
         return len(self.sites)
@@ -103,7 +157,19 @@ class MPO:
         return iter(self.sites)
 
     def materialize(self) -> "MPO":
-        return self
+        """The same operator as plain site tensors, with the channel table dropped."""
+        return MPO(list(self.sites))
+
+    @classmethod
+    def from_terms(cls, n_sites: int, terms, *, cutoff: float | None = 1e-13, device=None) -> "MPO":
+        """The MPO of a sum of terms ``(coef, [(op, site), ...])`` -- rank-3 charge-leg
+        factors from ``local_op`` on increasing sites, or one invariant two-site operator on
+        ``(i, i + 1)`` (reference ``mps.py:3135-3342``; ``tenet.network.terms``).
+        ``cutoff`` drops split channels whose weight is below it relative to the operator."""
+        from tenet.network.terms import build_mpo
+
+        sites, table = build_mpo(n_sites, list(terms), tol=1e-13 if cutoff is None else cutoff, device=device)
+        return cls(sites, edges=table)
 
     @classmethod
     def from_w(cls, w, n_sites: int, *, phys: GradedSpace, bond: GradedSpace,

@@ -404,3 +404,10 @@ def allclose(a: SymmetricTensor, b: SymmetricTensor, rtol: float = 1e-5, atol: f
 
 
 __all__ += ["inner_device", "RecoupleProgram"]
+
+
+def block_sqrt(t: SymmetricTensor) -> SymmetricTensor:
+    """Elementwise square root of every block (reference ``ops/blocks.py``): on a diagonal
+    singular-value or density spectrum tensor, the spectrum's square root."""
+    require_device(t)
+    return SymmetricTensor(t.structure, torch.sqrt(t.data))
