@@ -80,9 +80,38 @@ def restarted_lanczos(op, aa, refine=None, max_restarts: int = 40, ncv: int = 8,
     return e, used
 
 
-def cpu_baseline(fl, w1, w2, fr, aa, budget_s: float = 15.0):
-    """The oracle's NumPy FP64 H_eff on the host cores; bounded sample of whole matvecs."""
-    import numpy as np
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _median_reps(fn, reps: int, budget_s: float):
+    """One warm-up call, then ``reps`` timed calls (fewer, but at least 5, when the warm-up
+    says ``reps`` would exceed ``budget_s``); returns (median seconds, reps, warm-up out)."""
+    import statistics
+
+    t0 = time.perf_counter()
+    out = fn()
+    first = time.perf_counter() - t0
+    n = reps if first * reps <= budget_s else max(5, int(budget_s / max(first, 1e-6)))
+    times = []
+    for _ in range(n):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times), n, out
+
+
+def cpu_baseline(fl, w1, w2, fr, aa, reps: int = 20, budget_s: float = 40.0):
+    """The oracle's NumPy FP64 H_eff on the host cores, two arms: every core of this
+    process's affinity set (BLAS threads = that count) and one core (BLAS threads = 1),
+    each a warm-up and the median of ``reps`` whole matvecs."""
+    from threadpoolctl import threadpool_limits
 
     sys.path.insert(0, str(ROOT))
     from oracle import u1_network as N
@@ -90,16 +119,18 @@ def cpu_baseline(fl, w1, w2, fr, aa, budget_s: float = 15.0):
 
     cores = len(os.sched_getaffinity(0))
     ofl, ow1, ow2, ofr, oaa = (to_oracle(t) for t in (fl, w1, w2, fr, aa))
-    t0 = time.perf_counter()
-    out = N.heff2(ofl, ow1, ow2, ofr, oaa)  # warm-up, also the full-size parity reference
-    first = time.perf_counter() - t0
-    reps = max(1, min(20, int(budget_s / max(first, 1e-3))))
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        N.heff2(ofl, ow1, ow2, ofr, oaa)
-    dt = (time.perf_counter() - t0) / reps
-    return {"value": 1.0 / dt, "unit": "matvecs/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} timed H_eff matvecs after 1 warm-up, NumPy float64, BLAS threads = {cores}"}, out.flat()
+    fn = lambda: N.heff2(ofl, ow1, ow2, ofr, oaa)  # noqa: E731
+    with threadpool_limits(limits=cores):
+        t_all, n_all, out = _median_reps(fn, reps, budget_s)
+    with threadpool_limits(limits=1):
+        t_one, n_one, _ = _median_reps(fn, reps, budget_s)
+    model = _cpu_model()
+    return {"value": 1.0 / t_all, "unit": "matvecs/s", "cores": cores, "kind": "port",
+            "cpu_model": model,
+            "sample": f"median of {n_all} whole H_eff matvecs after 1 warm-up, NumPy float64, "
+                      f"BLAS threads = {cores} (threadpoolctl)",
+            "single_core": {"value": 1.0 / t_one, "unit": "matvecs/s", "cores": 1,
+                            "sample": f"median of {n_one} matvecs after 1 warm-up, BLAS threads = 1"}}, out.flat()
 
 
 def main() -> int:
@@ -123,8 +154,8 @@ def main() -> int:
                     help="serve workload: mp (worker processes), single (one process), serial (one-at-a-time FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
-              "(the H_eff of one bond runs on one device; see the sharding note in DESIGN.md)",
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the H_eff of one bond runs on one device)",
               file=sys.stderr)
         return 2
 
@@ -194,20 +225,24 @@ def bench_heff2(args) -> int:
                                   "matvec_rel_err_vs_fp64": float((got - ref).norm() / ref.norm())}
         log(f"arm fp64_blocks: {dt * 1e3:.3f} ms/matvec")
 
-    # GEMM kernel timing over a timed pass of the headline arm
-    head = "fp32" if "fp32" in arms else arms[0]
-    ev: dict = {}
-    for _ in range(args.steps):
-        ops_[head].apply_data(aa.structure, aa.data, events=ev)
-    torch.cuda.synchronize()
+    # GEMM kernel timing over a timed pass of an arm: HIP events recorded on the stream
+    # the kernels are launched on, around each launch
     aux = ("recouple", "reduce", "entry", "exit")
-    per_step = {k: (sum(e0.elapsed_time(e1) for e0, e1, _ in v) / len(v), v[0][2]) for k, v in ev.items()
-                if v and not str(k).startswith(aux)}
-    reduce_ms = {str(k): sum(e0.elapsed_time(e1) for e0, e1, _ in v) / len(v) for k, v in ev.items()
-                 if v and str(k).startswith(aux)}
-    big = [k for k in per_step if per_step[k][1] >= 0.05 * flops]
-    gemm_ms = sum(per_step[k][0] for k in big)
-    gemm_flops = sum(per_step[k][1] for k in big)
+
+    def kernel_times(p):
+        ev: dict = {}
+        for _ in range(args.steps):
+            ops_[p].apply_data(aa.structure, aa.data, events=ev)
+        torch.cuda.synchronize()
+        per_step = {k: (sum(e0.elapsed_time(e1) for e0, e1, _ in v) / len(v), v[0][2]) for k, v in ev.items()
+                    if v and not str(k).startswith(aux)}
+        reduce_ms = {str(k): sum(e0.elapsed_time(e1) for e0, e1, _ in v) / len(v) for k, v in ev.items()
+                     if v and str(k).startswith(aux)}
+        big = [k for k in per_step if per_step[k][1] >= 0.05 * flops]
+        return per_step, reduce_ms, big, sum(per_step[k][0] for k in big), sum(per_step[k][1] for k in big)
+
+    head = "fp32" if "fp32" in arms else arms[0]
+    per_step, reduce_ms, big, gemm_ms, gemm_flops = kernel_times(head)
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
     pmc = PMC_TRAFFIC.get((args.sites, args.bond, head))
     traffic = None
@@ -225,6 +260,19 @@ def bench_heff2(args) -> int:
                 # slice sum + scatter into the vector's blocks + cast), plain reductions
                 "aux_passes_avg_ms": reduce_ms,
                 "traffic_source": pmc["source"] if pmc else None}
+
+    # the split-BF16x3 arm against its own peak: each FP32 product is three BF16 MFMA
+    # products (hi*hi + hi*lo + lo*hi), so the BF16 rate is 3x the FP32-equivalent one
+    roofline_bf16 = None
+    if "bf16x3" in arms:
+        b_steps, _, b_big, b_ms, b_flops = kernel_times("bf16x3")
+        eq = b_flops / (b_ms * 1e-3) / 1e12
+        roofline_bf16 = {"bound": "mfma", "achieved": 3 * eq, "peak": PEAK_TFLOPS["bf16x3"], "unit": "TFLOP/s",
+                         "frac": 3 * eq / PEAK_TFLOPS["bf16x3"],
+                         "fp32_equivalent_tflops": eq, "fp32_equivalent_frac_of_fp32_peak": eq / PEAK_TFLOPS["fp32"],
+                         "avg_launch_ms": {str(k): b_steps[k][0] for k in b_big},
+                         "kernel": "gemm_kernel (tn_gemm_grouped, bf16x3): the two H_eff GEMM launches; "
+                                   "achieved counts 3 BF16 products per FP32 flop"}
 
     # the headline arm's matvec captured as a hipGraph and replayed on resident buffers
     gm = GraphedMatvec(ops_[head], aa.structure)
@@ -259,7 +307,8 @@ def bench_heff2(args) -> int:
         cpu, ref_cpu = cpu_baseline(fl, w1, w2, fr, aa)
         got64 = ref.to("cpu").numpy()
         results["fp64"]["rel_err_vs_oracle"] = float(np.linalg.norm(got64 - ref_cpu) / np.linalg.norm(ref_cpu))
-        log(f"cpu baseline {cpu['value']:.3f} matvecs/s")
+        log(f"cpu baseline {cpu['value']:.3f} matvecs/s ({cpu['cores']} cores), "
+            f"{cpu['single_core']['value']:.3f} matvecs/s (1 core), {cpu['cpu_model']}")
 
     # the headline is the faster of the two launch modes of the same kernels
     best = f"{head}_graph" if results[f"{head}_graph"]["matvecs_per_s"] > results[head]["matvecs_per_s"] else head
@@ -280,6 +329,9 @@ def bench_heff2(args) -> int:
         "config": {
             "workload": f"two-site H_eff, U(1) Heisenberg L={args.sites}, D={args.bond} centre bond "
                         f"(22 odd-charge sectors, Gaussian sigma=6), MPO bond 5, phys dim 2",
+            "sites": "random N(0,1) MPS sites, each normalised (not canonized: the operator's block "
+                     "structure and flops are those of a canonized chain, only the values differ), "
+                     "environments folded from them and normalised",
             "aa_elements": aa.structure.size,
             "flops_per_matvec": flops,
             "launch": "hipGraph replay of the captured matvec" if best != head else "eager launches",
@@ -289,6 +341,7 @@ def bench_heff2(args) -> int:
         "speedup_fp32_vs_cpu": (v / cpu["value"]) if cpu else None,
         "energy": energy,
         "roofline": roofline,
+        "roofline_bf16x3": roofline_bf16,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
