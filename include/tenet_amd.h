@@ -86,6 +86,24 @@ int tn_recouple_cast(int src_dtype, int dst_dtype, const void* src, void* dst,
                      const int64_t* jobs, const int64_t* term_src, const double* term_coeff,
                      const int64_t* tiles, int64_t ntiles, void* stream);
 
+/* Host-side plan compilation (no device call): the job table of one tn_recouple launch from
+ * its n destination regions, replacing the per-term block loop of the reference's plan
+ * application (ops/permutation.py:254-262, ops/repartition.py's apply) with one compiled
+ * table.  Region j: dst_off[j], and rank dims (rows j of the n x rank tables shape,
+ * dst_strides, src_strides, destination C order, unit dims allowed); its terms are
+ * term_src[term_ptr[j] .. term_ptr[j+1]).  Writes jobs (n x TN_JOB_WORDS, unit dims dropped,
+ * dims contiguous on both sides merged, the mode word chosen) and tile_counts (n), and
+ * stats = {elements, element-terms, total tiles}; lin_tile / lin_small_tile / lds_tile /
+ * small_program_tiles are the kernel's tile sizes and the small-program threshold below
+ * which the small tiles are used.  Returns 1 (a write outside dst_size), 2 (a region
+ * spanning 2^31 elements), 3 (a read outside src_size), 4 (rank above TN_MAX_RANK after
+ * merging) or 5 (bad tile sizes), with tn_last_error set. */
+int tn_recouple_plan(int64_t n, int64_t rank, const int64_t* dst_off, const int64_t* shape,
+                     const int64_t* dst_strides, const int64_t* src_strides, const int64_t* term_ptr,
+                     const int64_t* term_src, int64_t src_size, int64_t dst_size, int64_t lin_tile,
+                     int64_t lin_small_tile, int64_t lds_tile, int64_t small_program_tiles,
+                     int64_t* jobs, int64_t* tile_counts, int64_t* stats);
+
 /* For every problem p: C_p = sum_s op(A_s) op(B_s) with op = transpose when trans_a/b,
  * written to C, or to the workspace W when the problem is flagged as a split-K partial.
  * A/B/C/W are base pointers; every offset in the tables is in elements.  bm x bn is the
@@ -167,14 +185,16 @@ int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t* jobs, int6
  *   [5] cols (0: x holds A, 1: x holds A^T) [6] u_off [7] s_off [8] vh_off
  *   [9] inner sweeps per block-pair visit
  * tn_svd_jacobi_round runs round `round` of the block tournament: workgroup i takes block
- * pair slot wgs[2i+1] of problem wgs[2i]; span = the largest L + k of the launch (LDS is
- * 2 w span doubles); first != 0 initialises Q = I on load (round 0 of the first sweep);
+ * pair slot wgs[2i+1] of problem wgs[2i]; LDS is 2 w span doubles, at least 2 w (L + k)
+ * for every problem of the launch with nb > 2 and (k + k % 2) (L + k) for one with nb = 2
+ * (held by one block pair, which converges inside the launch); first != 0 initialises
+ * Q = I on load (round 0 of the first sweep);
  * rotations[i] receives the rotation count of its last inner sweep.  A pair rotates when
  * |<x_a,x_b>| > tol * |x_a| |x_b|.
  * tn_svd_jacobi_finish writes, per problem, S (k, descending), U (rows x k, row-major) and
  * Vh (k x cols, row-major) of the thin SVD. */
 #define TN_SVD_PROB_WORDS 10
-#define TN_SVD_MAX_W 32
+#define TN_SVD_MAX_W 64
 #define TN_SVD_MAX_K 4096
 int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int64_t nwgs, int64_t span,
                         double* x, double* q, int32_t* rotations, int w, int round, int first,

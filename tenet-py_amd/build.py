@@ -12,7 +12,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "tenet" / "libtenet_amd.so"
-SOURCES = ["recouple.hip", "gemm.hip", "blas1.hip", "svd.hip", "krylov.hip", "batch.hip"]
+SOURCES = ["recouple.hip", "gemm.hip", "blas1.hip", "svd.hip", "krylov.hip", "batch.hip", "plan.cpp"]
 ARCH = os.environ.get("TENET_OFFLOAD_ARCH", "gfx950")
 
 

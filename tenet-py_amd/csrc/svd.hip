@@ -9,8 +9,8 @@
 // Vectors are grouped in blocks of w.  One launch is one round of a circle-method
 // tournament over the blocks of every problem: each workgroup takes one block pair, stages
 // its 2w vectors and their Q rows in LDS, runs inner tournaments over the 2w vectors
-// (every pair once per inner sweep; the w pairs of a round side by side, one 32-lane
-// group each), and writes both back.
+// (every pair once per inner sweep; the pairs of a round side by side, one 16-lane group
+// each, at most 64 groups), and writes both back.
 // A problem whose vectors fit in one block pair converges inside a single launch.
 // Each workgroup reports the rotations of its last inner sweep, so the host stops when a
 // whole sweep reports none.
@@ -40,18 +40,20 @@ __device__ inline double dpp_move(double v) {
   return __builtin_bit_cast(double, i32x2{lo, hi});
 }
 
-__device__ inline double half_sum(double v) {
-  // all-reduce within each 32-lane half of the wave: lane ^ 1 and lane ^ 2 (quad_perm),
+__device__ inline double row_sum(double v) {
+  // all-reduce within each 16-lane row of the wave: lane ^ 1 and lane ^ 2 (quad_perm),
   // the other quad of 8 lanes (row_half_mirror), the other 8 of 16 (row_mirror) -- DPP
-  // moves, no LDS crossbar -- then the other row of 16 (one permute).  Every step adds
-  // two partial sums that both partners hold, so all 32 lanes end with the same bits.
+  // moves only, no LDS crossbar.  Every step adds two partial sums that both partners
+  // hold, so all 16 lanes end with the same bits.
   v += dpp_move<0xB1>(v);
   v += dpp_move<0x4E>(v);
   v += dpp_move<0x141>(v);
   v += dpp_move<0x140>(v);
-  v += __shfl_xor(v, 16, 32);
   return v;
 }
+
+// lanes per vector pair: one 16-lane DPP row (a wave works on four pairs at once)
+constexpr int kPairLanes = 16;
 
 // circle method over n (even) players: round r in [0, n-1), pair p in [0, n/2)
 __device__ inline void pair_of(int n, int r, int p, int& a, int& b) {
@@ -83,9 +85,14 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
   int bi, bj;
   pair_of(nb, round, slot, bi, bj);
   const int n2 = 2 * w;
-  double* xs = lds;                                  // n2 x L
-  double* qs = lds + static_cast<size_t>(n2) * L;    // n2 x k
-  for (int t = 0; t < n2; ++t) {
+  // a problem held by one block pair (bi = 0, bj = 1, vector t is vector t) stages and
+  // runs its inner tournament over its own k vectors (one zero vector when k is odd), not
+  // over the 2w slots: the zero slots would only cost LDS and dot products that never
+  // rotate
+  const int ne = nb == 2 ? min(n2, k + (k & 1)) : n2;
+  double* xs = lds;                                  // ne x L
+  double* qs = lds + static_cast<size_t>(ne) * L;    // ne x k
+  for (int t = 0; t < ne; ++t) {
     const int g = t < w ? bi * w + t : bj * w + (t - w);
     if (g < k) {
       const double* xr = x + xo + static_cast<int64_t>(g) * L;
@@ -98,32 +105,32 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
       for (int i = threadIdx.x; i < k; i += blockDim.x) qs[t * k + i] = 0.0;
     }
   }
-  // one pair per 32-lane group: a round's w pairs run side by side
-  const int gl = threadIdx.x & 31;
-  const int grp = threadIdx.x >> 5;
-  const int ngrp = blockDim.x >> 5;
+  // one pair per 16-lane group: a round's pairs run side by side
+  const int gl = threadIdx.x % kPairLanes;
+  const int grp = threadIdx.x / kPairLanes;
+  const int ngrp = blockDim.x / kPairLanes;
   const double tol2 = tol * tol;
   int last = 0;
   for (int it = 0; it < inner_sweeps; ++it) {
     if (threadIdx.x == 0) rot_s = 0;
     __syncthreads();
     int mine = 0;
-    for (int r = 0; r < n2 - 1; ++r) {
-      for (int p = grp; p < w; p += ngrp) {
+    for (int r = 0; r < ne - 1; ++r) {
+      for (int p = grp; p < ne / 2; p += ngrp) {
         int a, b;
-        pair_of(n2, r, p, a, b);
+        pair_of(ne, r, p, a, b);
         double* xa = xs + a * L;
         double* xb = xs + b * L;
         double al = 0.0, be = 0.0, ga = 0.0;
-        for (int i = gl; i < L; i += 32) {
+        for (int i = gl; i < L; i += kPairLanes) {
           const double u = xa[i], v = xb[i];
           al = fma(u, u, al);
           be = fma(v, v, be);
           ga = fma(u, v, ga);
         }
-        al = half_sum(al);
-        be = half_sum(be);
-        ga = half_sum(ga);
+        al = row_sum(al);
+        be = row_sum(be);
+        ga = row_sum(ga);
         // uniform within the group: every lane of it holds the same sums
         if (al > 0.0 && be > 0.0 && ga * ga > tol2 * al * be) {
           // t = tan(theta), |theta| <= pi/4, zeroes <x_a, x_b>; applied as
@@ -140,14 +147,14 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
           const double qd = 1.0 / (rr * den);
           const double s = t * den * qd;
           const double cm1 = -tt * rr * qd;
-          for (int i = gl; i < L; i += 32) {
+          for (int i = gl; i < L; i += kPairLanes) {
             const double u = xa[i], v = xb[i];
             xa[i] = u + fma(cm1, u, -s * v);
             xb[i] = v + fma(cm1, v, s * u);
           }
           double* qa = qs + a * k;
           double* qb = qs + b * k;
-          for (int i = gl; i < k; i += 32) {
+          for (int i = gl; i < k; i += kPairLanes) {
             const double u = qa[i], v = qb[i];
             qa[i] = u + fma(cm1, u, -s * v);
             qb[i] = v + fma(cm1, v, s * u);
@@ -163,7 +170,7 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
     __syncthreads();
     if (last == 0) break;
   }
-  for (int t = 0; t < n2; ++t) {
+  for (int t = 0; t < ne; ++t) {
     const int g = t < w ? bi * w + t : bj * w + (t - w);
     if (g >= k) continue;
     double* xr = x + xo + static_cast<int64_t>(g) * L;
@@ -319,14 +326,14 @@ extern "C" int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int
   if (nwgs <= 0) return 0;
   if (w < 1 || w > TN_SVD_MAX_W) return tn::fail("tn_svd_jacobi_round: block width out of range");
   if (nwgs > 0x7fffffff) return tn::fail("tn_svd_jacobi_round: too many workgroups");
-  // LDS per workgroup: 2w vectors of length L plus their 2w Q rows of length k, where
-  // span is the largest L + k of the launch
+  // LDS per workgroup: 2w span doubles, at least the 2w (L + k) of a tournament problem
+  // and the (k + k % 2) (L + k) of a problem held by one block pair
   const size_t bytes = static_cast<size_t>(2 * w) * static_cast<size_t>(span) * sizeof(double);
   if (span <= 0) return tn::fail("tn_svd_jacobi_round: empty launch span");
   if (int rc = allow_lds(reinterpret_cast<const void*>(jacobi_round), bytes, "tn_svd_jacobi_round"))
     return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int threads = w <= 2 ? 64 : 32 * w;
+  const int threads = w <= 4 ? 64 : (w >= 64 ? 1024 : 16 * w);  // one 16-lane group per pair
   hipLaunchKernelGGL(jacobi_round, dim3(static_cast<unsigned>(nwgs)), dim3(threads), bytes, s, probs,
                      wgs, x, q, rotations, w, round, first, tol, stop);
   return tn::check_launch("jacobi_round");

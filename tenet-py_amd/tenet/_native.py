@@ -24,7 +24,7 @@ SEG_WORDS = 6
 MPO_JOB_WORDS = 8
 MPO_TILE = 1024  # positions per tn_mpo_apply tile (256 threads x 4)
 SVD_PROB_WORDS = 10
-SVD_MAX_W = 32
+SVD_MAX_W = 64
 SVD_MAX_K = 4096
 
 _p = ctypes.c_void_p
@@ -38,6 +38,8 @@ _SIGNATURES = {
     "tn_device_cus": (_i32, [ctypes.POINTER(_i32)]),
     "tn_recouple": (_i32, [_i32, _p, _p, _p, _p, _p, _p, _i64, _p]),
     "tn_recouple_cast": (_i32, [_i32, _i32, _p, _p, _p, _p, _p, _p, _i64, _p]),
+    "tn_recouple_plan": (_i32, [_i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i64, _i64,
+                                _p, _p, _p]),
     "tn_gemm_grouped": (_i32, [_i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32, _p]),
     "tn_gemm_grouped_static": (_i32, [_i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32,
                                       _i64, _p, _p, _p, _p, _i64, _p]),
@@ -91,6 +93,26 @@ def lib() -> ctypes.CDLL:
             )
         _lib = load_library()
     return _lib
+
+
+_host = None
+
+
+def host_lib() -> ctypes.CDLL:
+    """The loaded library for its host-side entry points (plan compilation), which need no
+    device; raises when the library is missing."""
+    global _host
+    if _lib is not None:
+        return _lib
+    if _host is None:
+        _host = load_library()
+    return _host
+
+
+def check_host(rc: int, exc=ValueError) -> None:
+    """Raise ``exc`` with the library's message when a host-side entry point failed."""
+    if rc != 0:
+        raise exc(host_lib().tn_last_error().decode(errors="replace"))
 
 
 def check(rc: int) -> None:

@@ -124,102 +124,77 @@ class _DeviceCache:
 # ----------------------------------------------------------------------------- recouple
 
 
-def _normalize(shape, dstr, sstr):
-    """Drop unit dims and merge dims that are contiguous in both source and destination."""
-    dims = [(n, d, s) for n, d, s in zip(shape, dstr, sstr) if n != 1]
-    if not dims:
-        return [1], [1], [1]
-    merged = [list(dims[0])]
-    for n, d, s in dims[1:]:
-        pn, pd, ps = merged[-1]
-        if pd == d * n and ps == s * n:
-            merged[-1] = [pn * n, d, s]
-        else:
-            merged.append([n, d, s])
-    return [m[0] for m in merged], [m[1] for m in merged], [m[2] for m in merged]
+def _pad_regions(regions):
+    """``(dst_off, shape, dst_strides, src_strides, [(src_off, coeff), ...])`` regions as
+    the tables of ``RecoupleProgram.from_arrays`` (ranks padded with unit dims)."""
+    n = len(regions)
+    R = max((len(r[1]) for r in regions), default=0)
+    doff = np.fromiter((r[0] for r in regions), dtype=np.int64, count=n)
+    shp = np.ones((n, R), dtype=np.int64)
+    dstr = np.zeros((n, R), dtype=np.int64)
+    sstr = np.zeros((n, R), dtype=np.int64)
+    for j, (_, shape, ds, ss, _) in enumerate(regions):
+        r = len(shape)
+        shp[j, :r], dstr[j, :r], sstr[j, :r] = shape, ds, ss
+    counts = np.fromiter((len(r[4]) for r in regions), dtype=np.int64, count=n)
+    ptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=ptr[1:])
+    terms = [t for r in regions for t in r[4]]
+    tsrc = np.fromiter((t[0] for t in terms), dtype=np.int64, count=len(terms))
+    tco = np.fromiter(((t[1].real if isinstance(t[1], complex) else t[1]) for t in terms),
+                      dtype=np.float64, count=len(terms))
+    return doff, shp, dstr, sstr, ptr, tsrc, tco
 
 
 class RecoupleProgram:
     """A compiled ``tn_recouple`` launch.
 
     ``regions`` is a list of ``(dst_off, shape, dst_strides, src_strides, [(src_off,
-    coeff), ...])``; dims are given in destination order.
+    coeff), ...])``; dims are given in destination order.  ``from_arrays`` builds the same
+    program from those regions as tables (one row per region, ranks padded with unit dims;
+    terms as a CSR list ``term_ptr``/``term_src``/``term_coef``).
     """
 
     def __init__(self, regions, src_size: int, dst_size: int) -> None:
-        n = len(regions)
-        rows = []  # job words, one list per region (the table is built in one step)
-        tsrc, tco, meta = [], [], []
-        tb = 0
-        elements = moved = 0
-        pad = [0] * MAX_RANK
-        for doff, shape, dstr, sstr, terms in regions:
-            shp, ds, ss = _normalize(shape, dstr, sstr)
-            r = len(shp)
-            if r > MAX_RANK:
-                raise ValueError(f"recouple region of rank {r} exceeds {MAX_RANK}")
-            nelem = math.prod(shp)
-            mode = -1
-            if ss[-1] != 1 and ds[-1] == 1 and shp[-1] >= 16:
-                for k in range(r - 1):
-                    if ss[k] == 1 and shp[k] >= 16:
-                        mode = k
-                        break
-            nterms = len(terms)
-            fill = pad[: MAX_RANK - r]
-            rows.append([doff, tb, tb + nterms, r, *shp, *fill, *ds, *fill, *ss, *fill, mode, nelem])
-            for so, c in terms:
-                tsrc.append(so)
-                tco.append(float(c.real) if isinstance(c, complex) else float(c))
-            tb += nterms
-            meta.append((mode, shp, nelem))
-            elements += nelem
-            moved += nelem * nterms
-            # every access stays in bounds: checked here once, on the host
-            dspan = sum((x - 1) * y for x, y in zip(shp, ds))
-            if doff < 0 or doff + dspan >= dst_size:
-                raise ValueError("recouple region writes outside its destination buffer")
-            span = sum((x - 1) * y for x, y in zip(shp, ss))
-            # the kernel addresses elements within a region with 32-bit offsets
-            if max(span, dspan) >= 2**31:
-                raise ValueError("recouple region spans 2^31 elements or more")
-            for so, _ in terms:
-                if so < 0 or so + span >= src_size:
-                    raise ValueError("recouple term reads outside its source buffer")
-        self.elements, self.moved_terms = elements, moved
+        self._build(*_pad_regions(regions), src_size, dst_size)
+
+    @classmethod
+    def from_arrays(cls, dst_off, shape, dst_strides, src_strides, term_ptr, term_src, term_coef,
+                    src_size: int, dst_size: int) -> "RecoupleProgram":
+        prog = cls.__new__(cls)
+        prog._build(np.asarray(dst_off, dtype=np.int64), np.asarray(shape, dtype=np.int64),
+                    np.asarray(dst_strides, dtype=np.int64), np.asarray(src_strides, dtype=np.int64),
+                    np.asarray(term_ptr, dtype=np.int64), np.asarray(term_src, dtype=np.int64),
+                    np.asarray(term_coef, dtype=np.float64), src_size, dst_size)
+        return prog
+
+    def _build(self, doff, shp, dstr, sstr, ptr, tsrc, tco, src_size: int, dst_size: int) -> None:
+        n = len(doff)
+        c = np.ascontiguousarray
+        doff, shp, dstr, sstr, ptr, tsrc = (c(x, dtype=np.int64) for x in (doff, shp, dstr, sstr, ptr, tsrc))
+        jobs = np.empty((n, JOB_WORDS), dtype=np.int64)
+        counts = np.empty(n, dtype=np.int64)
+        stats = np.zeros(3, dtype=np.int64)
+        addr = lambda x: x.ctypes.data  # noqa: E731
+        # unit dims dropped, contiguous dims merged, mode chosen, every access checked
+        # against the buffers, tiles counted (the small-tile rule applied): one host call
+        _native.check_host(_native.host_lib().tn_recouple_plan(
+            n, shp.shape[1] if shp.ndim == 2 else 0, addr(doff), addr(shp), addr(dstr), addr(sstr), addr(ptr),
+            addr(tsrc), src_size, dst_size, _LIN_TILE, _LIN_SMALL_TILE, _LDS_T, _SMALL_PROGRAM_TILES,
+            addr(jobs), addr(counts), addr(stats)))
+        self.elements, self.moved_terms, total = (int(v) for v in stats)
         self.njobs = n
-        jobs = np.asarray(rows, dtype=np.int64).reshape(n, JOB_WORDS)
-
-        def large_tiles(mode, shp, nelem):
-            if mode < 0:
-                return -(-nelem // _LIN_TILE)
-            return (-(-shp[-1] // _LDS_T)) * (-(-shp[mode] // _LDS_T)) * (nelem // (shp[-1] * shp[mode]))
-
-        # a program of few large regions would leave most of the device idle: give its LDS
-        # jobs 32 x 32 tiles, four times the workgroups
-        # (mode word J + 8); its linear jobs likewise get 512-element tiles (mode word -2)
-        counts = np.array([large_tiles(*m) for m in meta], dtype=np.int64)
-        if int(counts.sum()) < _SMALL_PROGRAM_TILES:
-            e = _LDS_T // 2
-            for j, (mode, shp, nelem) in enumerate(meta):
-                if mode >= 0:
-                    jobs[j, 22] = mode + 8
-                    counts[j] = (-(-shp[-1] // e)) * (-(-shp[mode] // e)) * (nelem // (shp[-1] * shp[mode]))
-                else:
-                    jobs[j, 22] = -2
-                    counts[j] = -(-nelem // _LIN_SMALL_TILE)
         # tile table: (job, tile index within the job) for every job's tiles, in job order
-        total = int(counts.sum())
         starts = np.cumsum(counts) - counts
         tiles_arr = np.empty((total, 2), dtype=np.int64)
         tiles_arr[:, 0] = np.repeat(np.arange(n, dtype=np.int64), counts)
         tiles_arr[:, 1] = np.arange(total, dtype=np.int64) - np.repeat(starts, counts)
-        self.ntiles = len(tiles_arr)
+        self.ntiles = total
         self.src_size, self.dst_size = src_size, dst_size
         self._tables = _DeviceCache(
             jobs=jobs.reshape(-1),
-            tsrc=np.asarray(tsrc, dtype=np.int64),
-            tco=np.asarray(tco, dtype=np.float64),
+            tsrc=tsrc,
+            tco=c(tco, dtype=np.float64),
             tiles=tiles_arr.reshape(-1),
         )
 
@@ -251,30 +226,44 @@ class RecoupleProgram:
         )
 
 
+def _plan_arrays(S: TensorStructure, D: TensorStructure, perm, terms):
+    """The destination-region tables of a (perm, terms) block map between two flat
+    layouts: per target block its offset, shape and strides, the source strides permuted
+    into destination order, and its terms (source offsets, coefficients) in plan order."""
+    T = len(terms)
+    try:
+        t = np.array(terms, dtype=np.float64).reshape(T, 3)
+    except TypeError:  # complex coefficients: their real parts are the stored ones
+        t = np.array(terms, dtype=np.complex128).reshape(T, 3)
+    ti, tj, tc = t[:, 0].real.astype(np.int64), t[:, 1].real.astype(np.int64), t[:, 2].real
+    counts = np.bincount(tj, minlength=D.num_blocks) if T else np.zeros(D.num_blocks, np.int64)
+    if len(counts) != D.num_blocks or int((counts > 0).sum()) != D.num_blocks:
+        raise ValueError(
+            f"the plan fills {int((counts > 0).sum())} of {D.num_blocks} target blocks; the "
+            "provider's coefficients dropped terms"
+        )
+    order = np.argsort(tj, kind="stable")
+    ptr = np.zeros(D.num_blocks + 1, dtype=np.int64)
+    np.cumsum(counts, out=ptr[1:])
+    sl, dl = S.layout, D.layout
+    first = ti[order][ptr[:-1]]
+    sstr = sl.strides[first][:, list(perm)] if len(perm) else np.zeros((D.num_blocks, 0), np.int64)
+    return dl.offsets, dl.shapes, dl.strides, sstr, ptr, sl.offsets[ti[order]], tc[order]
+
+
 def regions_for_plan(S: TensorStructure, D: TensorStructure, perm, terms):
     """Destination regions of a (perm, terms) block map between two flat layouts."""
-    by_dst = defaultdict(list)
-    for i, j, c in terms:
-        by_dst[j].append((i, c))
-    if len(by_dst) != D.num_blocks:
-        raise ValueError(
-            f"the plan fills {len(by_dst)} of {D.num_blocks} target blocks; the provider's "
-            "coefficients dropped terms"
-        )
-    sl, dl = S.layout, D.layout
-    s_off = sl.offsets.tolist()
-    s_str = np.asarray(sl.strides)[:, list(perm)].tolist() if len(perm) else [[] for _ in s_off]
-    d_off = dl.offsets.tolist()
-    d_shp = dl.shapes.tolist()
-    d_str = dl.strides.tolist()
-    return [(d_off[j], tuple(d_shp[j]), tuple(d_str[j]), tuple(s_str[by_dst[j][0][0]]),
-             [(s_off[i], c) for i, c in by_dst[j]])
+    doff, shp, dstr, sstr, ptr, tsrc, tco = _plan_arrays(S, D, perm, terms)
+    d_off, d_shp, d_str, s_str = doff.tolist(), shp.tolist(), dstr.tolist(), sstr.tolist()
+    src, co = tsrc.tolist(), tco.tolist()
+    return [(d_off[j], tuple(d_shp[j]), tuple(d_str[j]), tuple(s_str[j]),
+             list(zip(src[ptr[j]:ptr[j + 1]], co[ptr[j]:ptr[j + 1]])))
             for j in range(D.num_blocks)]
 
 
 @lru_cache(maxsize=4096)
 def recouple_program(S: TensorStructure, D: TensorStructure, perm: tuple, terms: tuple):
-    return RecoupleProgram(regions_for_plan(S, D, perm, terms), S.size, D.size)
+    return RecoupleProgram.from_arrays(*_plan_arrays(S, D, perm, terms), S.size, D.size)
 
 
 def inplace_map(S: TensorStructure, D: TensorStructure, perm, terms):

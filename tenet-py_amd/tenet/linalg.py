@@ -42,7 +42,12 @@ class _MatrixLayout:
     def __init__(self, s: TensorStructure) -> None:
         lay = map_layout(s)
         self.sectors = lay.sectors
-        self.shapes = {c: lay.shape(c) for c in lay.sectors}
+        rows_of, cols_of = {c: [] for c in lay.sectors}, {c: [] for c in lay.sectors}
+        for c, _, off, ext in lay.rows:
+            rows_of[c].append((off, ext))
+        for c, _, off, ext in lay.cols:
+            cols_of[c].append((off, ext))
+        self.shapes = {c: (sum(e for _, e in rows_of[c]), sum(e for _, e in cols_of[c])) for c in lay.sectors}
         self.offsets = {}
         acc = 0
         for c in lay.sectors:
@@ -50,33 +55,36 @@ class _MatrixLayout:
             r, k = self.shapes[c]
             acc += r * k
         self.size = acc
-        bl = s.layout
-        b_shapes, b_strides, b_offs = bl.shapes.tolist(), bl.strides.tolist(), bl.offsets.tolist()
-        n_out = len(s.out_axes)
-        order = (*s.out_axes, *s.in_axes)
-        to_regions, from_regions = [], []
+        # one region per block: its matrix offset (row band x column band of its sector)
+        blk, moff = [], []
         for c, cells in lay.grid:
-            rb, cb = lay.row_bands(c), lay.col_bands(c)
-            ld = self.shapes[c][1]
-            base = self.offsets[c]
-            for (_, ro, dr), row in zip(rb, cells):
-                for (_, co, dc), i in zip(cb, row):
-                    shape = tuple(b_shapes[i][a] for a in order)
-                    blk = tuple(b_strides[i][a] for a in order)
-                    mat = [0] * len(order)
-                    st = ld
-                    for d in range(n_out - 1, -1, -1):
-                        mat[d] = st
-                        st *= shape[d]
-                    st = 1
-                    for d in range(len(order) - 1, n_out - 1, -1):
-                        mat[d] = st
-                        st *= shape[d]
-                    moff = base + ro * ld + co
-                    to_regions.append((moff, shape, tuple(mat), blk, [(b_offs[i], 1.0)]))
-                    from_regions.append((b_offs[i], shape, blk, tuple(mat), [(moff, 1.0)]))
-        self.to_prog = RecoupleProgram(to_regions, s.size, self.size)
-        self.from_prog = RecoupleProgram(from_regions, self.size, s.size)
+            ld, base = self.shapes[c][1], self.offsets[c]
+            for (ro, _), row in zip(rows_of[c], cells):
+                for (co, _), i in zip(cols_of[c], row):
+                    blk.append(i)
+                    moff.append(base + ro * ld + co)
+        blk = np.asarray(blk, dtype=np.int64)
+        moff = np.asarray(moff, dtype=np.int64)
+        ld = np.asarray([self.shapes[c][1] for c, cells in lay.grid for row in cells for _ in row],
+                        dtype=np.int64)
+        bl = s.layout
+        order = [*s.out_axes, *s.in_axes]
+        n_out = len(s.out_axes)
+        shape = bl.shapes[blk][:, order] if len(blk) else np.zeros((0, len(order)), np.int64)
+        bstr = bl.strides[blk][:, order] if len(blk) else np.zeros((0, len(order)), np.int64)
+        # matrix strides: rows (the output axes) of length ld, columns C-contiguous
+        mat = np.ones_like(shape)
+        for d in range(len(order) - 2, n_out - 1, -1):
+            mat[:, d] = mat[:, d + 1] * shape[:, d + 1]
+        if n_out:
+            mat[:, n_out - 1] = ld
+            for d in range(n_out - 2, -1, -1):
+                mat[:, d] = mat[:, d + 1] * shape[:, d + 1]
+        boff = bl.offsets[blk] if len(blk) else np.zeros(0, np.int64)
+        ptr = np.arange(len(blk) + 1, dtype=np.int64)
+        ones = np.ones(len(blk))
+        self.to_prog = RecoupleProgram.from_arrays(moff, shape, mat, bstr, ptr, boff, ones, s.size, self.size)
+        self.from_prog = RecoupleProgram.from_arrays(boff, shape, bstr, mat, ptr, moff, ones, self.size, s.size)
 
 
 @lru_cache(maxsize=1024)
@@ -254,11 +262,19 @@ def _lds_budget() -> int:
     return _SVD_LDS_BYTES
 
 
+def _single_pair(k: int, L: int) -> bool:
+    """Whether one workgroup holds the whole problem: its k vectors (k + 1 when k is odd)
+    of length L and their Q rows of length k in LDS, at most 2 * SVD_MAX_W vectors.  Such a
+    problem converges inside one launch."""
+    from tenet._native import SVD_MAX_W
+
+    return k <= 2 * SVD_MAX_W and ((k + (k & 1)) * (L + k) + k + 1) * 8 <= _lds_budget()
+
+
 def _svd_width(span: int, k: int) -> int:
-    """Vectors per block: the largest power of two <= 32 whose block pair (2w vectors of
-    length L and their Q rows of length k, span = L + k doubles each) fits in LDS, and no
-    wider than the smallest pair that holds all k vectors (an inner sweep of a block pair
-    is 2w - 1 dependent rounds, so a problem of 10 vectors runs 15 rounds, not 63)."""
+    """Vectors per block of the block tournament: the largest power of two <= 32 whose
+    block pair (2w vectors of length L and their Q rows of length k, span = L + k doubles
+    each) fits in LDS, and no wider than the smallest pair that holds all k vectors."""
     budget = _lds_budget()
     w = 32
     while w > 1 and (2 * w * span * 8 > budget or w >= k):
@@ -294,9 +310,12 @@ class _JacobiPlan:
                 regions.append((xo, (n, m), (m, 1), (1, n), [(src, 1.0)]))
             else:
                 regions.append((xo, (m, n), (n, 1), (n, 1), [(src, 1.0)]))
-            w = _svd_width(L + k, k)
-            nb = max(2, -(-k // w))
-            nb += nb % 2
+            if _single_pair(k, L):
+                w, nb = 0, 2  # the single-pair launch group (its width is set below)
+            else:
+                w = _svd_width(L + k, k)
+                nb = max(2, -(-k // w))
+                nb += nb % 2
             inner = _SVD_SINGLE_SWEEPS if nb == 2 else _SVD_INNER_SWEEPS
             rows.append([k, L, xo, qo, nb, int(cols), uo, so, vo, inner])
             widths.append(w)
@@ -315,6 +334,11 @@ class _JacobiPlan:
             sub = [r for r, ww in zip(rows, widths) if ww == w]
             wgs = [(p, slot) for p, r in enumerate(sub) for slot in range(r[4] // 2)]
             span = max(r[0] + r[1] for r in sub)
+            if w == 0:
+                # every problem held by one workgroup, in one launch: blocks of half the
+                # largest k (vector t is vector t), LDS for the largest (k + k % 2)(L + k)
+                w = max(1, max(-(-r[0] // 2) for r in sub))
+                span = -(-max((r[0] + (r[0] & 1)) * (r[0] + r[1]) for r in sub) // (2 * w))
             # rotate while |cos(x_a, x_b)| > tol: 8 eps sqrt(L) sits above the rounding
             # noise of a length-L dot product, so converged pairs stop rotating on noise
             tol = 8 * 2.220446049250313e-16 * max(1.0, max(r[1] for r in sub) ** 0.5)

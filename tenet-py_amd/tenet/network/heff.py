@@ -301,7 +301,6 @@ class _FusedChain:
         self.mpo_host = (jobs, ins, outs, coef)
         self.t1_size = s1.size
         self.mpo_njobs = len(jobs)
-        tiles = np.concatenate(tiles) if tiles else np.zeros((0, 2), np.int64)
         self.mpo_ntiles = len(tiles)
         self.mpo_tables = _DeviceCache(
             jobs=np.asarray(jobs, dtype=np.int64).reshape(-1),
@@ -348,6 +347,13 @@ class _FusedChain:
         return out
 
 
+def _sector_rows(s: TensorStructure) -> tuple:
+    """Per block of ``s`` (in block order) the space sector of every axis."""
+    from tenet.structure import _axis_sectors
+
+    return _axis_sectors(s)
+
+
 def _mpo_jobs(inputs, outputs, w12, src_size, dst_size):
     """``tn_mpo_apply`` tables from per-(a, s) channel lists.  ``inputs[(a, s)]`` and
     ``outputs[(a, s)]`` hold ``(offset, row_stride, dense_index, R, S)`` per channel slab;
@@ -370,15 +376,20 @@ def _mpo_jobs(inputs, outputs, w12, src_size, dst_size):
         ins.extend((i[0], i[1]) for i in ilist)
         outs.extend((o[0], o[1]) for o in olist)
         coef.extend(cm.reshape(-1).tolist())
-        nt = -(-(R * S) // MPO_TILE)
-        tiles.append(np.stack([np.full(nt, j, np.int64), np.arange(nt, dtype=np.int64)], 1))
+        tiles.append(-(-(R * S) // MPO_TILE))
         for off, rs, _, r_, s_ in ilist:
             if r_ != R or s_ != S or off < 0 or off + (R - 1) * rs + S > src_size:
                 raise ValueError("MPO pass reads outside its source")
         for off, rs, _, _, _ in olist:
             if off < 0 or off + (R - 1) * rs + S > dst_size:
                 raise ValueError("MPO pass writes outside its destination")
-    return jobs, ins, outs, coef, tiles
+    # tile table: (job, tile index within the job), every job's tiles in job order
+    per = np.asarray(tiles, dtype=np.int64)
+    total = int(per.sum())
+    table = np.empty((total, 2), dtype=np.int64)
+    table[:, 0] = np.repeat(np.arange(len(per), dtype=np.int64), per)
+    table[:, 1] = np.arange(total, dtype=np.int64) - np.repeat(np.cumsum(per) - per, per)
+    return jobs, ins, outs, coef, table
 
 
 def exit_regions(spec, slices, dst_shift: int = 0, problem_shift: int = 0):
@@ -416,14 +427,13 @@ class _SectorChain:
         w12 = np.einsum("xPpm,mQqy->xPQpqy", w1, w2)
         L = s_in.layout
         # ---- entry: aa blocks grouped by r ----
+        sec_in, sec_fr, sec_fl = (_sector_rows(x) for x in (s_in, fr_s, fl_s))
         rows_r: dict = {}
-        for i, key in enumerate(s_in.block_order):
-            a, p, q, r = s_in.axis_sectors(key)
+        for i, (a, p, q, r) in enumerate(sec_in):
             rows_r.setdefault(r, []).append(i)
         # ---- FR stacked per r: d_r x N_r, columns (y, s) blocks ----
         cols_r: dict = {}
-        for i, key in enumerate(fr_s.block_order):
-            r, y, sg = fr_s.axis_sectors(key)
+        for i, (r, y, sg) in enumerate(sec_fr):
             cols_r.setdefault(r, []).append(i)
         xs_base, x_rowoff, xs_regions, acc = {}, {}, [], 0
         frs_base, fr_coloff, n_r, frs_regions, facc = {}, {}, {}, [], 0
@@ -469,12 +479,10 @@ class _SectorChain:
         self.gemm1 = GemmProgram(problems1, max(acc, 1), max(facc, 1), max(t1_acc, 1), tile=(64, 64))
         # ---- FL stacked per B: K_B x d_B, rows (a, x) blocks; output blocks per B ----
         fl_rows: dict = {}
-        for i, key in enumerate(fl_s.block_order):
-            a, x, B = fl_s.axis_sectors(key)
+        for i, (a, x, B) in enumerate(sec_fl):
             fl_rows.setdefault(B, []).append(i)
         out_cols: dict = {}
-        for j, key in enumerate(s_in.block_order):
-            B, P, Q, sg = s_in.axis_sectors(key)
+        for j, (B, P, Q, sg) in enumerate(sec_in):
             out_cols.setdefault(B, []).append(j)
         fls_base, fl_rowoff, k_b, fls_regions, lacc = {}, {}, {}, [], 0
         t3_base, out_coloff, n_b, t3_acc = {}, {}, {}, 0
@@ -510,11 +518,10 @@ class _SectorChain:
         # ---- MPO pass: t1 (r-stacked) -> t3 (B-stacked), one job per (a, s) ----
         fr_by_r_s: dict = {}
         for i in [i for ids in cols_r.values() for i in ids]:
-            r, y, sg = fr_s.axis_sectors(fr_s.block_order[i])
+            r, y, sg = sec_fr[i]
             fr_by_r_s.setdefault(r, []).append((i, y, sg))
         inputs: dict = {}
-        for i, key in enumerate(s_in.block_order):
-            a, p, q, r = s_in.axis_sectors(key)
+        for i, (a, p, q, r) in enumerate(sec_in):
             if r not in t1_base:
                 continue
             da, dp, dq, _ = (int(v) for v in L.shapes[i])
@@ -531,10 +538,10 @@ class _SectorChain:
         for B in order3:
             N = n_b[B]
             for fi in fl_rows.get(B, ()):
-                a, x, _ = fl_s.axis_sectors(fl_s.block_order[fi])
+                a, x, _ = sec_fl[fi]
                 da, mx, _ = (int(v) for v in fl_s.layout.shapes[fi])
                 for j in out_cols[B]:
-                    _, P, Q, sg = s_in.axis_sectors(s_in.block_order[j])
+                    _, P, Q, sg = sec_in[j]
                     _, dP, dQ, ds = (int(v) for v in L.shapes[j])
                     for ix in range(mx):
                         for iP in range(dP):
@@ -545,7 +552,6 @@ class _SectorChain:
         jobs, ins, outs, coef, tiles = _mpo_jobs(inputs, outputs, w12, max(t1_acc, 1), max(t3_acc, 1))
         self.mpo_host = (jobs, ins, outs, coef)
         self.mpo_njobs = len(jobs)
-        tiles = np.concatenate(tiles) if tiles else np.zeros((0, 2), np.int64)
         self.mpo_ntiles = len(tiles)
         self.mpo_tables = _DeviceCache(
             jobs=np.asarray(jobs, dtype=np.int64).reshape(-1),

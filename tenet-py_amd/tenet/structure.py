@@ -194,9 +194,33 @@ def _axis_sectors(s: TensorStructure) -> tuple:
 
 
 @cache
+def _axis_index(s: TensorStructure) -> np.ndarray:
+    """Per block and axis, the position of the block's sector in that leg's space (the
+    spaces of a pattern and of every structure sharing it list the same sectors in the
+    same order), as an int64 (blocks x ndim) table."""
+    p = _pattern(s)
+    if p is not s:
+        return _axis_index(p)
+    pos = [{a: j for j, (a, _) in enumerate(leg.space.sectors)} for leg in s.legs]
+    rows = _axis_sectors(s)
+    return np.array([[pm[a] for pm, a in zip(pos, secs)] for secs in rows],
+                    dtype=np.int64).reshape(len(rows), s.ndim)
+
+
+def _block_shapes(s: TensorStructure) -> np.ndarray:
+    """Block shapes as an int64 (blocks x ndim) table: each leg's degeneracies gathered
+    at the blocks' sector positions."""
+    idx = _axis_index(s)
+    out = np.empty(idx.shape, dtype=np.int64)
+    for d, leg in enumerate(s.legs):
+        degs = np.fromiter((m for _, m in leg.space.sectors), dtype=np.int64, count=len(leg.space.sectors))
+        out[:, d] = degs[idx[:, d]] if len(idx) else 0
+    return out
+
+
+@cache
 def _shapes(s: TensorStructure) -> tuple:
-    maps = [leg.space._map for leg in s.legs]  # sector -> degeneracy, per axis
-    return tuple(tuple(m.get(a, 0) for m, a in zip(maps, secs)) for secs in _axis_sectors(s))
+    return tuple(map(tuple, _layout(s).shapes.tolist()))
 
 
 @dataclass(frozen=True)
@@ -217,7 +241,7 @@ class FlatLayout:
 
 @cache
 def _layout(s: TensorStructure) -> FlatLayout:
-    shapes = np.array(_shapes(s), dtype=np.int64).reshape(-1, s.ndim)
+    shapes = _block_shapes(s)
     sizes = np.prod(shapes, axis=1, dtype=np.int64) if len(shapes) else np.zeros(0, np.int64)
     offsets = np.zeros(len(sizes), dtype=np.int64)
     if len(sizes) > 1:

@@ -211,14 +211,14 @@ def test_product_from_dense_refuses_a_non_invariant_array():
 
 def test_su2_heisenberg_mpo_is_invariant_and_reproduces_s_dot_s():
     from tenet.dense import to_dense
-    from tenet.network.models import SU2_END, SU2_START, su2_heisenberg_mpo, su2_heisenberg_w
+    from tenet.network.models import SU2_END, SU2_START, su2_heisenberg_mpo_w, su2_heisenberg_w
 
     w = su2_heisenberg_w()
     sp = np.array([[0.0, 1.0], [0.0, 0.0]])
     ss = 0.25 * np.kron(np.diag([1.0, -1.0]), np.diag([1.0, -1.0])) + 0.5 * (np.kron(sp, sp.T) + np.kron(sp.T, sp))
     two = np.einsum("Ppa,aQq->PQpq", w[SU2_START, :, :, 2:], w[2:, :, :, SU2_END]).reshape(4, 4)
     np.testing.assert_allclose(two, ss, atol=1e-15)
-    mpo = su2_heisenberg_mpo(4, device="cpu")  # from_dense refuses a non-invariant W
+    mpo = su2_heisenberg_mpo_w(4, device="cpu")  # from_dense refuses a non-invariant W
     np.testing.assert_allclose(to_dense(mpo[1]), w, atol=1e-14)
 
 

@@ -56,7 +56,10 @@ def _run(shapes, seed=0, make=None):
 
 
 def test_jacobi_svd_mixed_batch():
-    shapes = [(1, 1), (5, 7), (7, 5), (40, 9), (9, 40), (64, 64), (70, 66), (130, 200), (300, 280), (3, 0)]
+    # up to (90, 90), (101, 99): one workgroup holds the whole problem (odd k: one zero
+    # vector); (110, 110), (130, 200), (300, 280): block tournaments
+    shapes = [(1, 1), (5, 7), (7, 5), (40, 9), (9, 40), (64, 64), (70, 66), (65, 67), (77, 75), (33, 33),
+              (90, 90), (101, 99), (110, 110), (130, 200), (300, 280), (3, 0)]
     mats, parts = _run(shapes)
     _check(mats, parts)
 
