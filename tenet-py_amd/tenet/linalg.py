@@ -334,6 +334,7 @@ class _JacobiPlan:
             sub = [r for r, ww in zip(rows, widths) if ww == w]
             wgs = [(p, slot) for p, r in enumerate(sub) for slot in range(r[4] // 2)]
             span = max(r[0] + r[1] for r in sub)
+            single = w == 0
             if w == 0:
                 # every problem held by one workgroup, in one launch: blocks of half the
                 # largest k (vector t is vector t), LDS for the largest (k + k % 2)(L + k)
@@ -343,9 +344,21 @@ class _JacobiPlan:
             # noise of a length-L dot product, so converged pairs stop rotating on noise
             tol = 8 * 2.220446049250313e-16 * max(1.0, max(r[1] for r in sub) ** 0.5)
             self.groups.append(dict(
-                w=w, span=span, tol=tol, rounds=max(r[4] for r in sub) - 1, nwgs=len(wgs),
+                w=w, span=span, tol=tol, rounds=max(r[4] for r in sub) - 1, nwgs=len(wgs), single=single,
                 tables=_DeviceCache(probs=np.asarray(sub, dtype=np.int64).reshape(-1),
                                     wgs=np.asarray(wgs, dtype=np.int32).reshape(-1))))
+
+
+_SIDE_STREAMS: dict = {}
+
+
+def _side_stream(dev: torch.device) -> torch.cuda.Stream:
+    """One extra stream per device (default priority) for independent SVD launch groups."""
+    key = dev.index if dev.index is not None else torch.cuda.current_device()
+    st = _SIDE_STREAMS.get(key)
+    if st is None:
+        st = _SIDE_STREAMS[key] = torch.cuda.Stream(device=dev)
+    return st
 
 
 def _jacobi_plan(ml: _MatrixLayout) -> _JacobiPlan:
@@ -378,21 +391,32 @@ def _svd_parts(packed) -> dict:
     # one convergence flag per launch group, set on the device after a sweep that rotated
     # nothing; sweeps are queued in batches and the host reads the flags once per batch
     stop = torch.zeros(len(plan.groups), dtype=torch.int32, device=dev)
+    # the launch groups touch disjoint problems: the one of sectors held by one workgroup
+    # (which converges inside its first launch) runs on a second stream beside the block
+    # tournaments, so the SVD waits for the longer of the two rather than their sum; the
+    # side stream starts after the gather and the zeroed counters and flags
+    main = torch.cuda.current_stream(dev)
+    side = _side_stream(dev) if len(plan.groups) > 1 else None
+    if side is not None:
+        side.wait_stream(main)
+    streams = [side.cuda_stream if (side is not None and g["single"]) else stream for g in plan.groups]
     global _SVD_LAST_SWEEPS
     done = 0
     while True:
         batch = min(_SVD_SWEEP_BATCH if done == 0 else _SVD_SWEEP_BATCH // 2, _SVD_MAX_SWEEPS - done)
         for sweep in range(done, done + batch):
-            for gi, (g, rot) in enumerate(zip(plan.groups, rots)):
+            for gi, (g, rot, st) in enumerate(zip(plan.groups, rots, streams)):
                 flag = stop[gi:]
                 for r in range(g["rounds"]):
                     _native.check(lib.tn_svd_jacobi_round(
                         g["tb"]["probs"].data_ptr(), g["tb"]["wgs"].data_ptr(), g["nwgs"], g["span"],
                         x.data_ptr(), q.data_ptr(), rot[r * g["nwgs"]:].data_ptr(), g["w"], r,
-                        int(sweep == 0 and r == 0), g["tol"], flag.data_ptr(), stream))
+                        int(sweep == 0 and r == 0), g["tol"], flag.data_ptr(), st))
                 _native.check(lib.tn_svd_jacobi_converged(rot.data_ptr(), g["rounds"] * g["nwgs"],
-                                                          flag.data_ptr(), stream))
+                                                          flag.data_ptr(), st))
         done += batch
+        if side is not None:
+            main.wait_stream(side)
         if bool(stop.all()):
             break
         if done >= _SVD_MAX_SWEEPS:
