@@ -61,7 +61,8 @@ extern "C" {
 /* GEMM problem table: TN_PROB_WORDS int64 per problem (one output block, or one K slice
  * of one when the host splits K):
  *   [0] M [1] N [2] c_off [3] ldc [4] seg_begin [5] seg_end
- *   [6] 0 = write to C, 1 = write to the workspace W (split-K partial)  [7] 0
+ *   [6] 0 = write to C, 1 = write to the workspace W (split-K partial)
+ *   [7] 0 (tn_gemm_grouped_rows: the problem's first entry in its row table)
  * Segment table: TN_SEG_WORDS int64 per K segment:
  *   [0] a_off [1] b_off [2] K [3] lda [4] ldb [5] 0
  * Tile table: 4 int32 per output tile: (problem, tile row, tile col, the problem's first
@@ -113,6 +114,20 @@ int tn_gemm_grouped(int precision, int dtype, int trans_a, int trans_b, const vo
                     const void* B, void* C, void* W, const int64_t* problems,
                     const int64_t* segs, const int32_t* tiles, int64_t ntiles, int bm, int bn,
                     int grid, void* stream);
+
+/* tn_gemm_grouped with A's rows gathered through a row table, replacing the to_matrices
+ * gather of the left operand (map_view.py:274-278) that precedes the per-sector matmul of
+ * ops/map.py:160-161 when a sector matrix's rows lie in several blocks: element (m, k) of
+ * problem p's segment s is A[seg.a_off + a_rows[problem word 7 + m] + k] (trans_a = 0).
+ * A is FP64 or FP32 (a_dtype; FP64 is rounded to FP32 as it is staged, the precision
+ * staging of an FP32 GEMM fused with the gather); B (row-major, trans_b = 0), C and W are
+ * FP32; precision is TN_PREC_FP32 or TN_PREC_BF16X3; one workgroup per tile (bm x bn one of
+ * 64x64, 128x64, 64x128, 128x128).  Problem word 7 must index a_rows so that every row of
+ * the problem has an entry; byte offsets within a segment are 32-bit. */
+int tn_gemm_grouped_rows(int precision, int a_dtype, const void* A, const int64_t* a_rows,
+                         const void* B, void* C, void* W, const int64_t* problems,
+                         const int64_t* segs, const int32_t* tiles, int64_t ntiles, int bm, int bn,
+                         void* stream);
 
 /* No counterpart in the reference: tn_gemm_grouped for callers that cannot allocate (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The

@@ -46,11 +46,17 @@ struct Traits<TN_PREC_FP64> {
   static constexpr int BK = 16;
   static constexpr int PAD = 2;
 };
+#ifndef TN_FP32_BK
+#define TN_FP32_BK 32
+#endif
+#ifndef TN_PIPE_WAVES_PER_EU
+#define TN_PIPE_WAVES_PER_EU 4
+#endif
 template <>
 struct Traits<TN_PREC_FP32> {
   using T = float;
   using Acc = f32x4;
-  static constexpr int BK = 32;
+  static constexpr int BK = TN_FP32_BK;
   static constexpr int PAD = 4;
 };
 template <>
@@ -459,10 +465,13 @@ struct PipeSmem<TN_PREC_BF16X3, TA, TB, BM, BN> : Smem<TN_PREC_BF16X3, BM, BN> {
   static constexpr bool BMN = false;
 };
 
-template <int PREC, bool TA, bool TB, int BM, int BN>
+template <int PREC, bool TA, bool TB, int BM, int BN, typename AT = typename Traits<PREC>::T, bool AROWS = false>
 struct Stage {
   using T = typename Traits<PREC>::T;
   using V4 = T __attribute__((ext_vector_type(4)));
+  using V4A = AT __attribute__((ext_vector_type(4)));
+  static_assert(!AROWS || !TA, "a row table gathers k-contiguous rows of A");
+  static_assert(sizeof(AT) == sizeof(T) || !TA, "a wider A than the arithmetic is read k-contiguous");
   using SM_ = PipeSmem<PREC, TA, TB, BM, BN>;
   static constexpr bool AMN = SM_::AMN, BMN = SM_::BMN;
   static constexpr int BK = Traits<PREC>::BK;
@@ -472,7 +481,8 @@ struct Stage {
   static constexpr bool QA_EXACT = BM * QK % kThreads == 0;
   static constexpr bool QB_EXACT = BN * QK % kThreads == 0;
   struct Regs {
-    V4 a[QA], b[QB];
+    V4A a[QA];
+    V4 b[QB];
   };
   // per quad: byte offset in the segment, row (or first row) and k (or first k) in the
   // tile, and how many of its rows lie inside the problem (0: none, 4: all)
@@ -480,9 +490,13 @@ struct Stage {
   int am[QA], bn[QB];
   int akq[QA], bkq[QB];
   int arows[QA], brows[QB];
+  int64_t abase[QA];  // AROWS: the quad's row start from the row table (elements)
   uint32_t astr, bstr;
 
-  __device__ __forceinline__ void positions(int tid, int m0, int n0, int M, int N) {
+  // rt: the problem's row table (AROWS), rt[m] = element offset of row m of A from the
+  // segment's a_off; read once per tile, only for rows inside the problem
+  __device__ __forceinline__ void positions(int tid, int m0, int n0, int M, int N,
+                                            const int64_t* __restrict__ rt = nullptr) {
 #pragma unroll
     for (int r = 0; r < QA; ++r) {
       const int q = r * kThreads + tid;
@@ -496,6 +510,7 @@ struct Stage {
       akq[r] = kq;
       const int in = M - (m0 + m);
       arows[r] = !live || in <= 0 ? 0 : (AMN ? (in < 4 ? in : 4) : 4);
+      if constexpr (AROWS) abase[r] = arows[r] > 0 ? rt[m0 + m] : 0;
     }
 #pragma unroll
     for (int r = 0; r < QB; ++r) {
@@ -520,8 +535,10 @@ struct Stage {
 #pragma unroll
     for (int r = 0; r < QA; ++r) {
       const int m = am[r] < 0 ? 0 : am[r];
-      const int64_t e = TA ? (int64_t)akq[r] * lda + (m0 + m) : (int64_t)(m0 + m) * lda + akq[r];
-      ao[r] = static_cast<uint32_t>(e * sizeof(T));
+      int64_t e;
+      if constexpr (AROWS) e = abase[r] + akq[r];
+      else e = TA ? (int64_t)akq[r] * lda + (m0 + m) : (int64_t)(m0 + m) * lda + akq[r];
+      ao[r] = static_cast<uint32_t>(e * sizeof(AT));
     }
 #pragma unroll
     for (int r = 0; r < QB; ++r) {
@@ -530,29 +547,33 @@ struct Stage {
       bo[r] = static_cast<uint32_t>(e * sizeof(T));
     }
   }
-  static __device__ __forceinline__ T load1(__amdgpu_buffer_rsrc_t d, uint32_t off) {
-    if constexpr (sizeof(T) == 8)
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(d, off, 0, 0));
+  template <typename E>
+  static __device__ __forceinline__ E load1(__amdgpu_buffer_rsrc_t d, uint32_t off) {
+    if constexpr (sizeof(E) == 8)
+      return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(d, off, 0, 0));
     else
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(d, off, 0, 0));
+      return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b32(d, off, 0, 0));
   }
   // 4 consecutive elements in memory (4 k of one row, or 4 rows at one k); `valid` of them
   // lie inside the problem.  Elements outside read through an offset beyond the
   // descriptor's range, which the hardware returns as zero.
-  static __device__ __forceinline__ V4 contig(__amdgpu_buffer_rsrc_t d, uint32_t off, int valid) {
-    V4 v;
+  template <typename E>
+  static __device__ __forceinline__ E __attribute__((ext_vector_type(4)))
+  contig(__amdgpu_buffer_rsrc_t d, uint32_t off, int valid) {
+    typedef E V4E __attribute__((ext_vector_type(4)));
+    V4E v;
     if (valid >= 4) {
-      if constexpr (sizeof(T) == 8) {
+      if constexpr (sizeof(E) == 8) {
         typedef double d2 __attribute__((ext_vector_type(2)));
         const d2 lo = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(d, off, 0, 0));
         const d2 hi = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(d, off + 16, 0, 0));
-        v = V4{lo[0], lo[1], hi[0], hi[1]};
+        v = V4E{lo[0], lo[1], hi[0], hi[1]};
       } else {
-        v = __builtin_bit_cast(V4, __builtin_amdgcn_raw_buffer_load_b128(d, off, 0, 0));
+        v = __builtin_bit_cast(V4E, __builtin_amdgcn_raw_buffer_load_b128(d, off, 0, 0));
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = load1(d, j < valid ? off + j * (uint32_t)sizeof(T) : 0x80000000u);
+      for (int j = 0; j < 4; ++j) v[j] = load1<E>(d, j < valid ? off + j * (uint32_t)sizeof(E) : 0x80000000u);
     }
     return v;
   }
@@ -561,32 +582,32 @@ struct Stage {
                                                int kq, int kleft) {
     V4 v;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = load1(d, (ok && kq + j < kleft) ? off + j * str : 0x80000000u);
+    for (int j = 0; j < 4; ++j) v[j] = load1<T>(d, (ok && kq + j < kleft) ? off + j * str : 0x80000000u);
     return v;
   }
-  __device__ __forceinline__ void fetch(Regs& rg, const T* Ap, const T* Bp, int kleft) const {
+  __device__ __forceinline__ void fetch(Regs& rg, const AT* Ap, const T* Bp, int kleft) const {
     const __amdgpu_buffer_rsrc_t da = __builtin_amdgcn_make_buffer_rsrc((void*)Ap, (short)0, 0x7fffffff, 0x00020000);
     const __amdgpu_buffer_rsrc_t db = __builtin_amdgcn_make_buffer_rsrc((void*)Bp, (short)0, 0x7fffffff, 0x00020000);
 #pragma unroll
     for (int r = 0; r < QA; ++r) {
-      if (AMN) {
-        rg.a[r] = contig(da, ao[r], akq[r] < kleft ? arows[r] : 0);
-      } else if (TA) {
+      if constexpr (AMN) {
+        rg.a[r] = contig<AT>(da, ao[r], akq[r] < kleft ? arows[r] : 0);
+      } else if constexpr (TA) {
         rg.a[r] = strided(da, ao[r], astr, arows[r] > 0, akq[r], kleft);
       } else {
         const int kv = kleft - akq[r];
-        rg.a[r] = contig(da, ao[r], arows[r] > 0 ? (kv < 4 ? (kv < 0 ? 0 : kv) : 4) : 0);
+        rg.a[r] = contig<AT>(da, ao[r], arows[r] > 0 ? (kv < 4 ? (kv < 0 ? 0 : kv) : 4) : 0);
       }
     }
 #pragma unroll
     for (int r = 0; r < QB; ++r) {
       if (BMN) {
-        rg.b[r] = contig(db, bo[r], bkq[r] < kleft ? brows[r] : 0);
+        rg.b[r] = contig<T>(db, bo[r], bkq[r] < kleft ? brows[r] : 0);
       } else if (!TB) {
         rg.b[r] = strided(db, bo[r], bstr, brows[r] > 0, bkq[r], kleft);
       } else {
         const int kv = kleft - bkq[r];
-        rg.b[r] = contig(db, bo[r], brows[r] > 0 ? (kv < 4 ? (kv < 0 ? 0 : kv) : 4) : 0);
+        rg.b[r] = contig<T>(db, bo[r], brows[r] > 0 ? (kv < 4 ? (kv < 0 ? 0 : kv) : 4) : 0);
       }
     }
   }
@@ -599,15 +620,18 @@ struct Stage {
         u4 h, l;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          h[j] = bf16_rne(rg.a[r][j]);
-          l[j] = bf16_rne(rg.a[r][j] - bf16_to_f32(h[j]));
+          const float x = static_cast<float>(rg.a[r][j]);
+          h[j] = bf16_rne(x);
+          l[j] = bf16_rne(x - bf16_to_f32(h[j]));
         }
         *reinterpret_cast<u4*>(&sm.ah[buf][am[r]][akq[r]]) = h;
         *reinterpret_cast<u4*>(&sm.al[buf][am[r]][akq[r]]) = l;
       } else if constexpr (AMN) {
         *static_cast<V4*>(__builtin_assume_aligned(&sm.a[buf][akq[r]][am[r]], 16)) = rg.a[r];
       } else {
-        *static_cast<V4*>(__builtin_assume_aligned(&sm.a[buf][am[r]][akq[r]], 16)) = rg.a[r];
+        // an FP64 A is rounded to the arithmetic's FP32 here, after its loads have landed
+        *static_cast<V4*>(__builtin_assume_aligned(&sm.a[buf][am[r]][akq[r]], 16)) =
+            __builtin_convertvector(rg.a[r], V4);
       }
     }
 #pragma unroll
@@ -695,13 +719,13 @@ __device__ __forceinline__ void mfma_step(const PipeSmem<PREC, TA, TB, BM, BN>& 
   }
 }
 
-template <int PREC, bool TA, bool TB, int BM, int BN>
-__global__ __launch_bounds__(kThreads, 4) void gemm_pipe_kernel(
-    const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ C, float* __restrict__ W,
+template <int PREC, bool TA, bool TB, int BM, int BN, typename AT = float, bool AROWS = false>
+__global__ __launch_bounds__(kThreads, TN_PIPE_WAVES_PER_EU) void gemm_pipe_kernel(
+    const AT* __restrict__ A, const float* __restrict__ B, float* __restrict__ C, float* __restrict__ W,
     const int64_t* __restrict__ probs, const int64_t* __restrict__ segs,
-    const int32_t* __restrict__ tiles, int ntiles) {
+    const int32_t* __restrict__ tiles, int ntiles, const int64_t* __restrict__ arow_tab = nullptr) {
   static_assert(PREC != TN_PREC_FP64, "FP64 runs gemm_kernel");
-  using St = Stage<PREC, TA, TB, BM, BN>;
+  using St = Stage<PREC, TA, TB, BM, BN, AT, AROWS>;
   constexpr int BK = Traits<PREC>::BK;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int SM = WM / 16, SN = WN / 16;
@@ -738,7 +762,8 @@ __global__ __launch_bounds__(kThreads, 4) void gemm_pipe_kernel(
     for (int j = 0; j < SN; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
 
   St st;
-  st.positions(tid, m0, n0, M, N);
+  // problem word 7 (AROWS): where the problem's rows start in the row table
+  st.positions(tid, m0, n0, M, N, AROWS ? arow_tab + pw[7] : nullptr);
   // fetch cursor: segment fs, k offset fk inside it (runs two K steps ahead of the MFMAs)
   int fk = 0;
   Seg fsg{};
@@ -757,7 +782,7 @@ __global__ __launch_bounds__(kThreads, 4) void gemm_pipe_kernel(
       }
     }
     if (fs >= se) return false;
-    const float* Ap = A + fsg.a_off + (TA ? (int64_t)fk * fsg.lda : fk);
+    const AT* Ap = A + fsg.a_off + (TA ? (int64_t)fk * fsg.lda : fk);
     const float* Bp = B + fsg.b_off + (TB ? fk : (int64_t)fk * fsg.ldb);
     st.fetch(rg, Ap, Bp, fsg.K - fk);
     fk += BK;
@@ -834,6 +859,30 @@ int launch_prec(int ta, int tb, int bm, int bn, const void* A, const void* B, vo
   if (ta && !tb) return launch_tile<PREC, true, false>(bm, bn, A, B, C, W, probs, segs, tiles, ntiles, grid, s);
   if (!ta && tb) return launch_tile<PREC, false, true>(bm, bn, A, B, C, W, probs, segs, tiles, ntiles, grid, s);
   return launch_tile<PREC, true, true>(bm, bn, A, B, C, W, probs, segs, tiles, ntiles, grid, s);
+}
+
+// A gathered through a row table (problem word 7 = the problem's first entry in it), read
+// in FP64 or FP32, B row-major (not transposed); FP32 or BF16x3 arithmetic, one workgroup
+// per tile.
+template <int PREC, typename AT>
+int launch_rows(int bm, int bn, const void* A, const int64_t* rows, const void* B, void* C, void* W,
+                const int64_t* probs, const int64_t* segs, const int32_t* tiles, int ntiles, hipStream_t s) {
+  const AT* a = static_cast<const AT*>(A);
+  const float* b = static_cast<const float*>(B);
+  float* c = static_cast<float*>(C);
+  float* w = static_cast<float*>(W);
+#define TN_ROWS_CASE(BM_, BN_)                                                                  \
+  if (bm == BM_ && bn == BN_) {                                                                 \
+    gemm_pipe_kernel<PREC, false, false, BM_, BN_, AT, true><<<ntiles, kThreads, 0, s>>>(      \
+        a, b, c, w, probs, segs, tiles, ntiles, rows);                                          \
+    return tn::check_launch("tn_gemm_grouped_rows");                                           \
+  }
+  TN_ROWS_CASE(64, 64)
+  TN_ROWS_CASE(128, 64)
+  TN_ROWS_CASE(64, 128)
+  TN_ROWS_CASE(128, 128)
+#undef TN_ROWS_CASE
+  return tn::fail("tn_gemm_grouped_rows: unsupported tile shape");
 }
 
 // ---- skinny problems: one output line per thread ----------------------------------------
@@ -965,6 +1014,27 @@ extern "C" int tn_gemm_grouped(int precision, int dtype, int trans_a, int trans_
   if (precision == TN_PREC_BF16X3)
     return launch_prec<TN_PREC_BF16X3>(trans_a, trans_b, bm, bn, A, B, C, W, problems, segs, tiles, nt, grid, s);
   return tn::fail("tn_gemm_grouped: unknown precision");
+}
+
+extern "C" int tn_gemm_grouped_rows(int precision, int a_dtype, const void* A, const int64_t* a_rows,
+                                    const void* B, void* C, void* W, const int64_t* problems,
+                                    const int64_t* segs, const int32_t* tiles, int64_t ntiles, int bm,
+                                    int bn, void* stream) {
+  if (ntiles <= 0) return 0;
+  if (ntiles > 0x7fffffffLL) return tn::fail("tn_gemm_grouped_rows: too many tiles");
+  if (a_rows == nullptr) return tn::fail("tn_gemm_grouped_rows: no row table");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nt = static_cast<int>(ntiles);
+  if (precision == TN_PREC_FP32) {
+    if (a_dtype == TN_F64) return launch_rows<TN_PREC_FP32, double>(bm, bn, A, a_rows, B, C, W, problems, segs, tiles, nt, s);
+    if (a_dtype == TN_F32) return launch_rows<TN_PREC_FP32, float>(bm, bn, A, a_rows, B, C, W, problems, segs, tiles, nt, s);
+  } else if (precision == TN_PREC_BF16X3) {
+    if (a_dtype == TN_F64) return launch_rows<TN_PREC_BF16X3, double>(bm, bn, A, a_rows, B, C, W, problems, segs, tiles, nt, s);
+    if (a_dtype == TN_F32) return launch_rows<TN_PREC_BF16X3, float>(bm, bn, A, a_rows, B, C, W, problems, segs, tiles, nt, s);
+  } else {
+    return tn::fail("tn_gemm_grouped_rows: FP32 or BF16x3 arithmetic only");
+  }
+  return tn::fail("tn_gemm_grouped_rows: A must be FP64 or FP32");
 }
 
 // ---- the entry point that allocates nothing ------------------------------------------

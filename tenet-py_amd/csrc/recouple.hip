@@ -55,11 +55,13 @@ __device__ __forceinline__ void linear_tile(const TS* __restrict__ src, TD* __re
     so[r] = static_cast<int32_t>(s + static_cast<int64_t>(rem) * sstr[0]);
     dof[r] = static_cast<int32_t>(d + static_cast<int64_t>(rem) * dstr[0]);
   }
-  // the first term's loads land in the accumulators; a further term (multi-term SU(2)
-  // jobs, summed K slices) loads into CH more registers
-  constexpr int CH = PER;
+  // acc = ((0 + c_0 v_0) + c_1 v_1) + ... in term order.  A single-term job loads straight
+  // into the accumulators; a multi-term job (SU(2) recouplings, summed K slices) issues
+  // the loads of up to TG terms together -- the terms of a group read through clamped
+  // indices, so every load is unconditional -- and only then accumulates them in order.
+  constexpr int TG = 4;
   T acc[PER];
-  if (tb < te) {
+  if (te - tb == 1) {
     const T c = static_cast<T>(tco[tb]);
     const TS* __restrict__ s0 = src + tsrc[tb];
 #pragma unroll
@@ -69,17 +71,23 @@ __device__ __forceinline__ void linear_tile(const TS* __restrict__ src, TD* __re
   } else {
 #pragma unroll
     for (int r = 0; r < PER; ++r) acc[r] = T(0);
-  }
-  for (int64_t t = tb + 1; t < te; ++t) {
-    const T c = static_cast<T>(tco[t]);
-    const TS* __restrict__ s0 = src + tsrc[t];
+    for (int64_t t0 = tb; t0 < te; t0 += TG) {
+      const int nt = te - t0 < TG ? static_cast<int>(te - t0) : TG;
+      T v[TG][PER];
 #pragma unroll
-    for (int r0 = 0; r0 < PER; r0 += CH) {
-      T v[CH];
+      for (int q = 0; q < TG; ++q) {
+        const TS* __restrict__ sq = src + tsrc[t0 + (q < nt ? q : 0)];
 #pragma unroll
-      for (int q = 0; q < CH; ++q) v[q] = static_cast<T>(s0[so[r0 + q]]);
+        for (int r = 0; r < PER; ++r) v[q][r] = static_cast<T>(sq[so[r]]);
+      }
 #pragma unroll
-      for (int q = 0; q < CH; ++q) acc[r0 + q] += c * v[q];
+      for (int q = 0; q < TG; ++q) {
+        if (q < nt) {
+          const T c = static_cast<T>(tco[t0 + q]);
+#pragma unroll
+          for (int r = 0; r < PER; ++r) acc[r] += c * v[q][r];
+        }
+      }
     }
   }
 #pragma unroll

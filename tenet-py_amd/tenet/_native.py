@@ -44,6 +44,7 @@ _SIGNATURES = {
     "tn_gemm_grouped_static": (_i32, [_i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _i32,
                                       _i64, _p, _p, _p, _p, _i64, _p]),
     "tn_static_scratch_bytes": (_i64, []),
+    "tn_gemm_grouped_rows": (_i32, [_i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _p]),
     "tn_gemm_skinny": (_i32, [_i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _i64, _p]),
     "tn_dot": (_i32, [_i32, _p, _p, _p, _i64, _p, _p, _p]),
     "tn_axpby": (_i32, [_i32, _i64, _d, _p, _i32, _p, _d, _p, _i32, _p, _p]),

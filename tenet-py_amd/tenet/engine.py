@@ -450,20 +450,41 @@ class GemmProgram:
     SKINNY_MAX_W = 8    # the kernel's register line width (kSkinnyMax)
     SKINNY_MAX_K = 64
 
+    ROWS_TILES = ((64, 64), (128, 64), (64, 128), (128, 128))  # tn_gemm_grouped_rows' tiles
+
     def __init__(self, problems, a_size, b_size, c_size, trans_a=False, trans_b=False,
-                 tile=None, split: bool = True, gather: bool = False) -> None:
+                 tile=None, split: bool = True, gather: bool = False, a_rows=None) -> None:
+        """``a_rows`` (optional, one integer array per problem): row m of problem p's A
+        starts ``a_rows[p][m]`` elements after each segment's ``a_off`` (k contiguous), so a
+        sector matrix whose rows lie in several blocks is read in place; the launch then
+        goes through ``tn_gemm_grouped_rows`` (FP32/BF16x3 arithmetic, A in FP64 or FP32)."""
         self.trans_a, self.trans_b = bool(trans_a), bool(trans_b)
         self.gather = bool(gather)
         self.a_size, self.b_size, self.c_size = a_size, b_size, c_size
         self.problems = problems  # unsplit, for merging into batched launches
         self.flops = 0
-        for M, N, c_off, ldc, sl in problems:
+        if a_rows is not None:
+            if self.trans_a or self.trans_b:
+                raise ValueError("a row-gathered A is read k-contiguous against a row-major B")
+            if len(a_rows) != len(problems):
+                raise ValueError("a_rows needs one row table per problem")
+            a_rows = [np.asarray(r, dtype=np.int64).reshape(-1) for r in a_rows]
+        self.a_rows = a_rows
+        for i, (M, N, c_off, ldc, sl) in enumerate(problems):
             if not gather and (c_off < 0 or c_off + (M - 1) * ldc + N > c_size):
                 raise ValueError("gemm problem writes outside the output buffer")
+            rows = a_rows[i] if a_rows is not None else None
+            if rows is not None and len(rows) != M:
+                raise ValueError("a row table needs one entry per row of its problem")
             for a_off, b_off, K, lda, ldb in sl:
-                a_hi = a_off + ((K - 1) * lda + M if self.trans_a else (M - 1) * lda + K)
+                if rows is None:
+                    a_lo = a_off
+                    a_hi = a_off + ((K - 1) * lda + M if self.trans_a else (M - 1) * lda + K)
+                else:
+                    a_lo = a_off + (int(rows.min()) if M else 0)
+                    a_hi = a_off + (int(rows.max()) if M else 0) + K
                 b_hi = b_off + ((N - 1) * ldb + K if self.trans_b else (K - 1) * ldb + N)
-                if a_off < 0 or b_off < 0 or a_hi > a_size or b_hi > b_size:
+                if a_lo < 0 or b_off < 0 or a_hi > a_size or b_hi > b_size:
                     raise ValueError("gemm segment reads outside its operand buffer")
                 # the kernel addresses a segment with 32-bit byte offsets from its base and
                 # marks rows outside a tile with offsets >= 2^31
@@ -481,8 +502,8 @@ class GemmProgram:
                 self.skinny = 1
             elif all(p[0] <= self.SKINNY_MAX_W for p in problems):
                 self.skinny = 0
-        if self.skinny is not None and gather:
-            self.skinny = None  # the skinny kernel writes C directly
+        if self.skinny is not None and (gather or a_rows is not None):
+            self.skinny = None  # the skinny kernel writes C directly and reads A by lda
         if self.skinny is not None:
             split = False
         bm, bn = self.tile
@@ -493,18 +514,23 @@ class GemmProgram:
         rows, segs, red = [], [], []
         self.slices = []
         ws = 0
-        for (M, N, c_off, ldc, sl), kt in zip(problems, Ks0):
+        # word 7 of a problem row: its first entry in the concatenated row table
+        rbase = np.zeros(len(problems), dtype=np.int64)
+        if a_rows is not None and len(problems):
+            rbase[1:] = np.cumsum([len(r) for r in a_rows])[:-1]
+        for pi, ((M, N, c_off, ldc, sl), kt) in enumerate(zip(problems, Ks0)):
+            rb = int(rbase[pi])
             nsplit = 1
             if factor > 1 and kt >= self.SPLIT_MIN_K:
                 nsplit = max(1, min(factor, kt // (self.SPLIT_MIN_K // 2)))
             if nsplit == 1 and not gather:
-                rows.append((M, N, c_off, ldc, len(segs), len(segs) + len(sl), 0, kt))
+                rows.append((M, N, c_off, ldc, len(segs), len(segs) + len(sl), 0, rb, kt))
                 segs.extend((*sg, 0) for sg in sl)
                 continue
             chunks = self._split_segments(sl, kt, nsplit) if nsplit > 1 else [list(sl)]
             offs = []
             for ch in chunks:
-                rows.append((M, N, ws, N, len(segs), len(segs) + len(ch), 1, sum(c[2] for c in ch)))
+                rows.append((M, N, ws, N, len(segs), len(segs) + len(ch), 1, rb, sum(c[2] for c in ch)))
                 segs.extend((*c, 0) for c in ch)
                 offs.append(ws)
                 ws += M * N
@@ -515,11 +541,15 @@ class GemmProgram:
         self.reduce = RecoupleProgram(red, ws, c_size) if red else None
         self._rows = rows
         self.nproblems = len(rows)
-        self._probs = np.asarray([r[:7] + (0,) for r in rows], dtype=np.int64).reshape(-1, PROB_WORDS)
+        self._probs = np.asarray([r[:8] for r in rows], dtype=np.int64).reshape(-1, PROB_WORDS)
         self._segs = np.asarray(segs, dtype=np.int64).reshape(-1, SEG_WORDS)
+        self._arows = (np.concatenate(a_rows) if a_rows else np.zeros(1, np.int64)) \
+            if a_rows is not None else None
+        if a_rows is not None and self.tile not in self.ROWS_TILES:
+            raise ValueError(f"a row-gathered A runs the MFMA tiles {self.ROWS_TILES}, not {self.tile}")
         self.Ms = [r[0] for r in rows]
         self.Ns = [r[1] for r in rows]
-        self.Ks = [r[7] for r in rows]
+        self.Ks = [r[8] for r in rows]
         self._by_tile: dict = {}
 
     def _split_segments(self, sl, kt, nsplit):
@@ -590,8 +620,9 @@ class GemmProgram:
             # one all-zero problem row past the real ones: the padding tiles of the XCD
             # layout point at it (M = N = 0, no segments: the workgroup exits at once)
             probs = np.vstack([self._probs, np.zeros((1, PROB_WORDS), np.int64)])
+            extra = {} if self._arows is None else {"arows": self._arows}
             d = _DeviceCache(probs=probs.reshape(-1), segs=self._segs.reshape(-1),
-                             tiles=tiles.reshape(-1))
+                             tiles=tiles.reshape(-1), **extra)
             d.ntiles = len(tiles)
             self._by_tile[key] = d
         return d.get(device), d.ntiles
@@ -604,6 +635,8 @@ class GemmProgram:
         reduction with its algorithmic bytes.  A gather program takes its workspace as C."""
         if self.nproblems == 0:
             return
+        if self.a_rows is not None:
+            return self._run_rows(A, B, C, precision, tile, grid, events)
         if not (A.dtype == B.dtype == C.dtype):
             raise TypeError("gemm: operand dtypes differ")
         c_need = self.ws_size if self.gather else self.c_size
@@ -668,6 +701,37 @@ class GemmProgram:
                 r1.record()
                 reduce_events.append((r0, r1, self.reduce.bytes_moved(C.element_size())))
 
+
+    def _run_rows(self, A, B, C, precision, tile, grid, events) -> None:
+        """The launch of a program with a row-gathered A (``tn_gemm_grouped_rows``)."""
+        if A.dtype not in (torch.float64, torch.float32) or B.dtype != torch.float32 or C.dtype != torch.float32:
+            raise TypeError("gemm with a row table: A FP64 or FP32, B and C FP32")
+        prec = precision or "fp32"
+        if prec not in ("fp32", "bf16x3"):
+            raise ValueError("gemm with a row table: FP32 or BF16x3 arithmetic")
+        tile = tuple(tile) if tile else self.tile
+        if grid != 0 or tile not in self.ROWS_TILES:
+            raise ValueError(f"gemm with a row table: one workgroup per tile, tiles {self.ROWS_TILES}")
+        c_need = self.ws_size if self.gather else self.c_size
+        if A.numel() < self.a_size or B.numel() < self.b_size or C.numel() < c_need:
+            raise ValueError("gemm: buffers smaller than the compiled plan")
+        t, ntiles = self._tables(tile, C.device)
+        if self.gather:
+            W = C
+        else:
+            W = torch.empty(self.ws_size, dtype=C.dtype, device=C.device) if self.ws_size else C
+        if events is not None:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+        _native.check(_native.lib().tn_gemm_grouped_rows(
+            _native.PRECISIONS[prec], _dtype_code(A.dtype), A.data_ptr(), t["arows"].data_ptr(),
+            B.data_ptr(), C.data_ptr(), W.data_ptr(), t["probs"].data_ptr(), t["segs"].data_ptr(),
+            t["tiles"].data_ptr(), ntiles, tile[0], tile[1], _native.stream_ptr()))
+        if events is not None:
+            e1.record()
+            events.append((e0, e1, self.flops))
+        if self.reduce is not None:
+            self.reduce.run(W, C)
 
     def run_static(self, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, precision: str | None = None,
                    tile=None) -> None:

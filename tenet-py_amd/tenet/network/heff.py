@@ -438,7 +438,7 @@ class _SectorChain:
         xs_base, x_rowoff, xs_regions, acc = {}, {}, [], 0
         frs_base, fr_coloff, n_r, frs_regions, facc = {}, {}, {}, [], 0
         t1_base, t1_acc = {}, 0
-        problems1 = []
+        problems1, problems1_rows, rows1 = [], [], []
         for r, blocks in rows_r.items():
             dr = int(L.shapes[blocks[0]][3])
             xs_base[r] = acc
@@ -468,6 +468,11 @@ class _SectorChain:
                 facc += dr * c0
             t1_base[r] = t1_acc
             problems1.append((m, n_r[r], t1_acc, n_r[r], [(xs_base[r], frs_base[r], dr, dr, n_r[r])]))
+            # the same problem reading x_r's rows where they lie in aa's blocks
+            problems1_rows.append((m, n_r[r], t1_acc, n_r[r], [(0, frs_base[r], dr, dr, n_r[r])]))
+            rows1.append(np.concatenate([int(L.offsets[i]) + dr * np.arange(int(np.prod(L.shapes[i][:3])),
+                                                                             dtype=np.int64)
+                                         for i in blocks]))
             t1_acc += m * n_r[r]
         self.xs_size, self.frs_size, self.t1_size = acc, facc, t1_acc
         # host specs kept beside the compiled programs: BatchedHeff2 merges them across bonds
@@ -477,6 +482,10 @@ class _SectorChain:
         # 64 x 64 workgroup tiles: the stacked sectors are few and uneven, and larger tiles
         # leave too few workgroups for the device (measured on the D = 2048 U(1) bond)
         self.gemm1 = GemmProgram(problems1, max(acc, 1), max(facc, 1), max(t1_acc, 1), tile=(64, 64))
+        # FP32 / BF16x3: step 1 reads aa's blocks in place through a row table and rounds
+        # them to FP32 as it stages them, so no entry pass runs
+        self.gemm1_rows = GemmProgram(problems1_rows, max(s_in.size, 1), max(facc, 1), max(t1_acc, 1),
+                                      tile=(64, 64), a_rows=rows1)
         # ---- FL stacked per B: K_B x d_B, rows (a, x) blocks; output blocks per B ----
         fl_rows: dict = {}
         for i, (a, x, B) in enumerate(sec_fl):
@@ -601,12 +610,16 @@ class _SectorChain:
             e1.record()
             events.setdefault(name, []).append((e0, e1, amount))
 
-        xs = torch.empty(max(self.xs_size, 1), dtype=wt, device=x.device)
-        timed("entry", lambda: self.entry.run(x, xs), self.entry.bytes_moved(xs.element_size()))
         t1 = torch.empty(max(self.t1_size, 1), dtype=wt, device=x.device)
-        self.gemm1.run(xs, frs, t1, precision,
-                       events=None if events is None else events.setdefault(0, []),
-                       reduce_events=None if events is None else events.setdefault("reduce0", []))
+        if wt == torch.float32 and x.dtype in (torch.float64, torch.float32):
+            self.gemm1_rows.run(x, frs, t1, precision,
+                                events=None if events is None else events.setdefault(0, []))
+        else:
+            xs = torch.empty(max(self.xs_size, 1), dtype=wt, device=x.device)
+            timed("entry", lambda: self.entry.run(x, xs), self.entry.bytes_moved(xs.element_size()))
+            self.gemm1.run(xs, frs, t1, precision,
+                           events=None if events is None else events.setdefault(0, []),
+                           reduce_events=None if events is None else events.setdefault("reduce0", []))
         t3 = torch.empty(max(self.t3_size, 1), dtype=wt, device=x.device)
         tb = self.mpo_tables.get(x.device)
         _native.check(_native.lib().tn_mpo_apply(

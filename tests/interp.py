@@ -29,11 +29,16 @@ def run_gemm(g, A, B, C):
     """``C_p = sum_s op(A_s) op(B_s)`` over the compiled problem and segment tables; flagged
     problems write the workspace (allocated here), then the program's own reduction runs."""
     W = np.zeros(max(g.ws_size, 1))
-    for M, N_, c_off, ldc, sb, se, flag, _ in g._probs:
+    for M, N_, c_off, ldc, sb, se, flag, rb in g._probs:
         acc = np.zeros((M, N_))
         for a_off, b_off, K, lda, ldb, _ in g._segs[sb:se]:
             m, n, k = np.arange(M)[:, None], np.arange(N_)[None, :], np.arange(K)
-            Am = A[a_off + k[None, :] * lda + m] if g.trans_a else A[a_off + m * lda + k[None, :]]
+            if g.a_rows is not None:  # tn_gemm_grouped_rows: row m at a_rows[word 7 + m]
+                Am = A[a_off + g._arows[rb + np.arange(M)][:, None] + k[None, :]]
+            elif g.trans_a:
+                Am = A[a_off + k[None, :] * lda + m]
+            else:
+                Am = A[a_off + m * lda + k[None, :]]
             Bm = B[b_off + n * ldb + k[:, None]] if g.trans_b else B[b_off + k[:, None] * ldb + n]
             acc += Am @ Bm
         out = W if flag else C

@@ -96,3 +96,31 @@ def test_sector_layout_with_split_k_matches_block_layout():
     assert rel_err(got.cpu().numpy(), ref.cpu().numpy()) < 1e-12
     got32 = Heff2(fl, w1, w2, fr, "fp32", layout="sector")(aa).data
     assert rel_err(got32.cpu().numpy(), ref.cpu().numpy()) < 2e-6
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_row_gathered_step_one_equals_the_entry_gather(prec):
+    """Step 1 of the FP32 / BF16x3 sector chain reads aa's FP64 blocks in place through its
+    row table and rounds them while staging (tn_gemm_grouped_rows); the same GEMM on the
+    entry pass's gathered, cast copy gives bit-identical t1 (same roundings, same
+    accumulation order), on a bond whose sectors span several 64-row tiles."""
+    from tenet.network import Heff2
+    from tenet.network.workloads import heff2_problem
+
+    fl, w1, w2, fr, aa = heff2_problem(20, 512, 4.0, seed=3, device="cuda")
+    op = Heff2(fl, w1, w2, fr, prec, layout="sector")
+    ch = op.chain(aa.structure)
+    frs, _ = ch.prepare(op._datas)
+    t1_rows = torch.full((max(ch.t1_size, 1),), float("nan"), dtype=torch.float32, device="cuda")
+    ch.gemm1_rows.run(aa.data, frs, t1_rows, prec)
+    xs = torch.empty(max(ch.xs_size, 1), dtype=torch.float32, device="cuda")
+    ch.entry.run(aa.data, xs)
+    t1 = torch.full_like(t1_rows, float("nan"))
+    ch.gemm1.run(xs, frs, t1, prec)
+    torch.cuda.synchronize()
+    assert not torch.isnan(t1_rows).any()
+    assert torch.equal(t1_rows, t1)
+    # an FP32 vector through the same row table
+    t1_32 = torch.full_like(t1_rows, float("nan"))
+    ch.gemm1_rows.run(aa.data.to(torch.float32), frs, t1_32, prec)
+    assert torch.equal(t1_32, t1)

@@ -13,16 +13,21 @@ from tests.helpers import rel_err, to_oracle
 from tests.interp import run_gemm, run_mpo, run_recouple
 
 
-def emulate(ch, fl, fr, aa):
+def emulate(ch, fl, fr, aa, rows: bool = False):
+    """The chain's tables on the host; ``rows``: step 1 reads aa's blocks through its row
+    table (the FP32 / BF16x3 form) instead of the entry-gathered x."""
     x = aa.data.double().numpy()
-    xs = np.zeros(max(ch.xs_size, 1))
-    run_recouple(ch.entry, x, xs)
     frs = np.zeros(max(ch.frs_size, 1))
     run_recouple(ch.fr_stack, fr.data.double().numpy(), frs)
     fls = np.zeros(max(ch.fls_size, 1))
     run_recouple(ch.fl_stack, fl.data.double().numpy(), fls)
     t1 = np.zeros(max(ch.t1_size, 1))
-    run_gemm(ch.gemm1, xs, frs, t1)
+    if rows:
+        run_gemm(ch.gemm1_rows, x, frs, t1)
+    else:
+        xs = np.zeros(max(ch.xs_size, 1))
+        run_recouple(ch.entry, x, xs)
+        run_gemm(ch.gemm1, xs, frs, t1)
     t3 = np.zeros(max(ch.t3_size, 1))
     run_mpo(*ch.mpo_host, t1, t3)
     ws = run_gemm(ch.gemm4, fls, t3, None)
@@ -60,9 +65,28 @@ def _chain(fl, w1, w2, fr, aa):
 def test_sector_chain_tables_reproduce_the_oracle(left, right, n_sites, dim, sigma, seed):
     fl, w1, w2, fr, aa = _problem(left, right, n_sites, dim, sigma, seed)
     ch = _chain(fl, w1, w2, fr, aa)
-    got = emulate(ch, fl, fr, aa)
     ref = N.heff2(*(to_oracle(t) for t in (fl, w1, w2, fr, aa)))
-    assert rel_err(got, ref.flat()) < 1e-12
+    for rows in (False, True):
+        got = emulate(ch, fl, fr, aa, rows=rows)
+        assert rel_err(got, ref.flat()) < 1e-12
+
+
+def test_row_table_addresses_every_row_of_aa_once():
+    """The step-1 row table of the FP32 chain: each row of every aa block appears exactly
+    once over the problems, in the entry gather's row order, and a row table shifted by one
+    row in a single block is caught by the oracle comparison."""
+    fl, w1, w2, fr, aa = _problem(9, 11, 20, 512, 4.0, 5)
+    ch = _chain(fl, w1, w2, fr, aa)
+    g = ch.gemm1_rows
+    starts = np.sort(g._arows)
+    L = aa.structure.layout
+    want = np.sort(np.concatenate([int(L.offsets[i]) + int(L.shapes[i][3]) * np.arange(int(np.prod(L.shapes[i][:3])))
+                                   for i in range(len(L.offsets))]))
+    assert np.array_equal(starts, want)
+    ref = N.heff2(*(to_oracle(t) for t in (fl, w1, w2, fr, aa))).flat()
+    g._arows = g._arows.copy()
+    g._arows[3] += int(L.shapes[0][3])  # a planted off-by-one-row
+    assert rel_err(emulate(ch, fl, fr, aa, rows=True), ref) > 1e-6
 
 
 def test_sector_chain_counts_the_same_work_as_the_block_chain():
