@@ -54,6 +54,8 @@ __device__ inline double row_sum(double v) {
 
 // lanes per vector pair: one 16-lane DPP row (a wave works on four pairs at once)
 constexpr int kPairLanes = 16;
+// global loads a thread keeps in flight while staging a block pair into LDS
+constexpr int kStage = 8;
 
 // circle method over n (even) players: round r in [0, n-1), pair p in [0, n/2)
 __device__ inline void pair_of(int n, int r, int p, int& a, int& b) {
@@ -92,17 +94,53 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
   const int ne = nb == 2 ? min(n2, k + (k & 1)) : n2;
   double* xs = lds;                                  // ne x L
   double* qs = lds + static_cast<size_t>(ne) * L;    // ne x k
-  for (int t = 0; t < ne; ++t) {
-    const int g = t < w ? bi * w + t : bj * w + (t - w);
-    if (g < k) {
-      const double* xr = x + xo + static_cast<int64_t>(g) * L;
-      const double* qr = q + qo + static_cast<int64_t>(g) * k;
-      for (int i = threadIdx.x; i < L; i += blockDim.x) xs[t * L + i] = xr[i];
-      for (int i = threadIdx.x; i < k; i += blockDim.x)
-        qs[t * k + i] = first ? (i == g ? 1.0 : 0.0) : qr[i];
+  // staging: the ne x L vector slab and the ne x k Q slab walked as flat index spaces,
+  // kStage loads of a thread issued before any of them is stored, so a workgroup has
+  // many global loads in flight instead of one vector row at a time
+  {
+    const int nx = ne * L, nq = ne * k;
+    for (int base = 0; base < nx; base += kStage * blockDim.x) {
+      double v[kStage];
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int e = base + u * blockDim.x + threadIdx.x;
+        const int t = e / L, i = e - (e / L) * L;
+        const int g = t < w ? bi * w + t : bj * w + (t - w);
+        // unconditional load (an element outside the problem reads its first element)
+        const bool ok = e < nx && g < k;
+        const double val = x[xo + (ok ? static_cast<int64_t>(g) * L + i : 0)];
+        v[u] = ok ? val : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int e = base + u * blockDim.x + threadIdx.x;
+        if (e < nx) xs[e] = v[u];
+      }
+    }
+    if (first) {  // Q starts as the identity
+      for (int e = threadIdx.x; e < nq; e += blockDim.x) {
+        const int t = e / k, i = e - (e / k) * k;
+        const int g = t < w ? bi * w + t : bj * w + (t - w);
+        qs[e] = g < k && i == g ? 1.0 : 0.0;
+      }
     } else {
-      for (int i = threadIdx.x; i < L; i += blockDim.x) xs[t * L + i] = 0.0;
-      for (int i = threadIdx.x; i < k; i += blockDim.x) qs[t * k + i] = 0.0;
+      for (int base = 0; base < nq; base += kStage * blockDim.x) {
+        double v[kStage];
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+          const int e = base + u * blockDim.x + threadIdx.x;
+          const int t = e / k, i = e - (e / k) * k;
+          const int g = t < w ? bi * w + t : bj * w + (t - w);
+          const bool ok = e < nq && g < k;
+          const double val = q[qo + (ok ? static_cast<int64_t>(g) * k + i : 0)];
+          v[u] = ok ? val : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+          const int e = base + u * blockDim.x + threadIdx.x;
+          if (e < nq) qs[e] = v[u];
+        }
+      }
     }
   }
   // one pair per 16-lane group: a round's pairs run side by side
@@ -170,13 +208,16 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
     __syncthreads();
     if (last == 0) break;
   }
-  for (int t = 0; t < ne; ++t) {
+  // write-back over the same flat index spaces (stores do not stall the issue)
+  for (int e = threadIdx.x; e < ne * L; e += blockDim.x) {
+    const int t = e / L, i = e - (e / L) * L;
     const int g = t < w ? bi * w + t : bj * w + (t - w);
-    if (g >= k) continue;
-    double* xr = x + xo + static_cast<int64_t>(g) * L;
-    double* qr = q + qo + static_cast<int64_t>(g) * k;
-    for (int i = threadIdx.x; i < L; i += blockDim.x) xr[i] = xs[t * L + i];
-    for (int i = threadIdx.x; i < k; i += blockDim.x) qr[i] = qs[t * k + i];
+    if (g < k) x[xo + static_cast<int64_t>(g) * L + i] = xs[e];
+  }
+  for (int e = threadIdx.x; e < ne * k; e += blockDim.x) {
+    const int t = e / k, i = e - (e / k) * k;
+    const int g = t < w ? bi * w + t : bj * w + (t - w);
+    if (g < k) q[qo + static_cast<int64_t>(g) * k + i] = qs[e];
   }
   if (threadIdx.x == 0) rot_out[blockIdx.x] = last;
 }

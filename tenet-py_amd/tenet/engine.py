@@ -84,24 +84,57 @@ def _dtype_code(dtype: torch.dtype) -> int:
 
 
 _TORCH_DTYPES = {np.dtype(np.int64): torch.int64, np.dtype(np.int32): torch.int32,
-                 np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32}
+                 np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+                 np.dtype(np.uint8): torch.uint8}
+
+
+def upload(raw: np.ndarray, device: torch.device) -> tuple:
+    """Host-to-device copy of a host array without stalling the host: the bytes go
+    through a pinned staging buffer (torch's caching host allocator, which keeps the
+    buffer until the copy has run) and the copy is queued on the current stream, after
+    the work already queued there.  A synchronous copy would wait for that work to finish,
+    so plan compilation on the host could never overlap the kernels in flight.  Returns
+    ``(device tensor, stream, event)``: the event marks the copy's completion for users on
+    other streams."""
+    raw = np.ascontiguousarray(raw)
+    if torch.device(device).type != "cuda":  # host-side table users (plan checks)
+        return torch.from_numpy(raw.copy()), None, None
+    pinned = torch.empty(raw.nbytes, dtype=torch.uint8, pin_memory=True)
+    pinned.numpy()[:] = raw.view(np.uint8).reshape(-1)
+    buf = pinned.to(device, non_blocking=True)
+    ev = torch.cuda.Event()
+    ev.record()
+    return buf.view(_TORCH_DTYPES[raw.dtype]), _native.stream_ptr(), ev
+
+
+def wait_upload(stream: int, ev) -> None:
+    """Order the current stream after an upload queued on ``stream``."""
+    if ev is not None and _native.stream_ptr() != stream:
+        torch.cuda.current_stream().wait_event(ev)
+
 
 
 class _DeviceCache:
     """Lazily uploaded copies of host tables, one per device.  All of a cache's tables
     travel in ONE host-to-device copy (packed at 256-byte aligned offsets into one byte
-    buffer) and are handed out as typed views of it."""
+    buffer) and are handed out as typed views of it.  The copy is queued on the stream
+    that first asks for the tables (``upload``); a later user on another stream is
+    ordered after it."""
 
     _ALIGN = 256
 
     def __init__(self, **arrays) -> None:
         self.host = arrays
         self._dev: dict = {}
+        self._ready: dict = {}
 
     def get(self, device: torch.device) -> dict:
         key = (device.type, device.index)
         d = self._dev.get(key)
-        if d is None:
+        if d is not None:
+            st, ev = self._ready[key]
+            wait_upload(st, ev)
+        else:
             parts, spans, off = [], [], 0
             for k, v in self.host.items():
                 a = np.ascontiguousarray(v)
@@ -113,11 +146,12 @@ class _DeviceCache:
             raw = np.zeros(max(off, self._ALIGN), dtype=np.uint8)
             for o, b in parts:
                 raw[o:o + b.size] = b
-            buf = torch.from_numpy(raw).to(device)
+            buf, st, ev = upload(raw, device)
             d = {}
             for k, o, n, dt in spans:
                 d[k] = buf[o:o + n * dt.itemsize].view(_TORCH_DTYPES[dt])
             self._dev[key] = d
+            self._ready[key] = (st, ev)
         return d
 
 

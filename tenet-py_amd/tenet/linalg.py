@@ -20,7 +20,7 @@ from functools import lru_cache
 import numpy as np
 import torch
 
-from tenet.engine import RecoupleProgram, require_device
+from tenet.engine import RecoupleProgram, require_device, upload, wait_upload
 from tenet.ops import repartition
 from tenet.plans import map_layout
 from tenet.space import IN, OUT, GradedSpace, Leg
@@ -465,11 +465,13 @@ def _diagonal(ss: TensorStructure, values: dict, scale: float = 1.0) -> Symmetri
     vals = torch.cat([values[c] for c in ml.sectors]) * scale
     cache = ml.__dict__.setdefault("_diag_index", {})
     key = (ref.device.type, ref.device.index)
-    idx = cache.get(key)
-    if idx is None:
-        idx = torch.as_tensor(np.concatenate([ml.offsets[c] + np.arange(ml.shapes[c][0]) * (ml.shapes[c][0] + 1)
-                                              for c in ml.sectors]).astype(np.int64), device=ref.device)
-        cache[key] = idx
+    got = cache.get(key)
+    if got is None:
+        got = cache[key] = upload(np.concatenate([ml.offsets[c] + np.arange(ml.shapes[c][0]) * (ml.shapes[c][0] + 1)
+                                                  for c in ml.sectors]).astype(np.int64), ref.device)
+    else:
+        wait_upload(got[1], got[2])
+    idx = got[0]
     if vals.numel() != idx.numel():
         raise ValueError("diagonal factor: value counts do not match the sector sizes")
     buf = torch.zeros(ml.size, dtype=ref.dtype, device=ref.device)

@@ -22,6 +22,8 @@ from tenet.engine import (
     inplace_map,
     recouple_program,
     require_device,
+    upload,
+    wait_upload,
 )
 from tenet.plans import (
     contraction_plan,
@@ -260,8 +262,10 @@ def _weights(structure, device):
     key = (structure, device)
     d = _WEIGHTS_DEV.get(key)
     if d is None:
-        d = _WEIGHTS_DEV[key] = torch.from_numpy(w).to(device)
-    return d
+        d = _WEIGHTS_DEV[key] = upload(w, device)
+    else:
+        wait_upload(d[1], d[2])
+    return d[0]
 
 
 def _work(device) -> torch.Tensor:
