@@ -151,7 +151,9 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=2.0)
     ap.add_argument("--max-sweeps", type=int, default=30)
     ap.add_argument("--serve-arms", default="mp,single,serial",
-                    help="serve workload: mp (worker processes), single (one process), serial (one-at-a-time FP64)")
+                    help="serve workload: mp (worker processes), mp_shared (worker processes, each batching "
+                         "its solves' Krylov steps), shared (one process, batched Krylov steps), single "
+                         "(one process, a stream per solve), serial (one-at-a-time FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -671,8 +673,8 @@ def bench_serve(args) -> int:
     against the same stream solved one at a time in FP64."""
     from tenet.network.serve import DMRGServer, MultiProcessServer, latency_report, poisson_requests
 
-    sizes = tuple(int(x) for x in os.environ.get("TENET_SERVE_SIZES", "32,64").split(","))
-    chis = tuple(int(x) for x in os.environ.get("TENET_SERVE_CHIS", "64,128").split(","))
+    sizes = tuple(int(x) for x in os.environ.get("TENET_SERVE_SIZES", "32,64,128").split(","))
+    chis = tuple(int(x) for x in os.environ.get("TENET_SERVE_CHIS", "128,256,512,1024").split(","))
     workers = int(os.environ.get("TENET_SERVE_WORKERS", "4"))
     reqs = poisson_requests(args.count, args.rate, sizes=sizes, chis=chis, seed=0,
                             max_sweeps=args.max_sweeps)
@@ -680,6 +682,9 @@ def bench_serve(args) -> int:
     # the multi-process arm first: its workers are spawned before this process uses the GPU
     arms = [("mp", f"continuous_fp32_{workers}proc", lambda: MultiProcessServer(workers=workers, max_active=8,
                                                                                 precision="fp32")),
+            ("mp_shared", f"shared_batches_fp32_{workers}proc",
+             lambda: MultiProcessServer(workers=workers, max_active=8, precision="fp32", mode="shared")),
+            ("shared", "shared_batches_fp32", lambda: DMRGServer(max_active=8, precision="fp32", mode="shared")),
             ("single", "continuous_fp32", lambda: DMRGServer(max_active=8, precision="fp32")),
             ("serial", "one_at_a_time_fp64", lambda: DMRGServer(max_active=1, precision="fp64"))]
     chosen = set(args.serve_arms.split(","))

@@ -198,7 +198,14 @@ int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t* jobs, int6
  * q + q_off holds its k x k orthogonal factor.  Problem table, TN_SVD_PROB_WORDS int64:
  *   [0] k [1] L [2] x_off [3] q_off [4] nb (even block count, blocks of w vectors)
  *   [5] cols (0: x holds A, 1: x holds A^T) [6] u_off [7] s_off [8] vh_off
- *   [9] inner sweeps per block-pair visit
+ *   [9] inner sweeps per block-pair visit  [10] the problem's slot in fro2
+ * tn_svd_jacobi_norms writes fro2[slot] = |A|_F^2 of every problem (run after x is filled,
+ * before the first round).  With null_scale > 0 a vector whose squared norm is at or below
+ * floor^2 = null_scale k^2 fro2 is numerically null: it still rotates, but its rotations
+ * do not count towards convergence, and tn_svd_jacobi_finish (same null_scale) leaves its
+ * row of the length-L factor zero unless asked to complete it (complete != 0: Gram-Schmidt
+ * of unit vectors against the rows already set, so U and Vh are isometries on every
+ * column).  null_scale = 0: plain one-sided Jacobi, every rotation counts.
  * tn_svd_jacobi_round runs round `round` of the block tournament: workgroup i takes block
  * pair slot wgs[2i+1] of problem wgs[2i]; LDS is 2 w span doubles, at least 2 w (L + k)
  * for every problem of the launch with nb > 2 and (k + k % 2) (L + k) for one with nb = 2
@@ -208,19 +215,23 @@ int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t* jobs, int6
  * |<x_a,x_b>| > tol * |x_a| |x_b|.
  * tn_svd_jacobi_finish writes, per problem, S (k, descending), U (rows x k, row-major) and
  * Vh (k x cols, row-major) of the thin SVD. */
-#define TN_SVD_PROB_WORDS 10
+#define TN_SVD_PROB_WORDS 11
 #define TN_SVD_MAX_W 64
 #define TN_SVD_MAX_K 4096
+int tn_svd_jacobi_norms(const int64_t* probs, int64_t nprobs, const double* x, double* fro2,
+                        void* stream);
 int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int64_t nwgs, int64_t span,
                         double* x, double* q, int32_t* rotations, int w, int round, int first,
-                        double tol, const int32_t* stop, void* stream);
+                        double tol, const int32_t* stop, const double* fro2, double null_scale,
+                        void* stream);
 /* *stop = 1 when all n rotation counts of a sweep (rotations of every round of one launch
  * group) are zero.  With `stop` passed to tn_svd_jacobi_round (NULL: always run), the host
  * queues several sweeps ahead and reads one flag instead of the counts of every sweep;
  * rounds past convergence return at once. */
 int tn_svd_jacobi_converged(const int32_t* rotations, int64_t n, int32_t* stop, void* stream);
 int tn_svd_jacobi_finish(const int64_t* probs, int64_t nprobs, const double* x, const double* q,
-                         double* U, double* S, double* Vh, void* stream);
+                         double* U, double* S, double* Vh, const double* fro2, double null_scale,
+                         int complete, void* stream);
 
 /* Ground eigenpair of the Lanczos tridiagonal matrix, on the device: replaces the
  * np.linalg.eigh(tri) of network/dmrg.py:216-222 (lanczos) and the host read of the Krylov

@@ -40,11 +40,11 @@ __device__ inline double dpp_move(double v) {
   return __builtin_bit_cast(double, i32x2{lo, hi});
 }
 
+// all-reduce within each 16-lane row of the wave: lane ^ 1 and lane ^ 2 (quad_perm), the
+// other quad of 8 lanes (row_half_mirror), the other 8 of 16 (row_mirror) -- DPP moves
+// only, no LDS crossbar.  Every step adds two partial sums that both partners hold, so all
+// 16 lanes end with the same bits.
 __device__ inline double row_sum(double v) {
-  // all-reduce within each 16-lane row of the wave: lane ^ 1 and lane ^ 2 (quad_perm),
-  // the other quad of 8 lanes (row_half_mirror), the other 8 of 16 (row_mirror) -- DPP
-  // moves only, no LDS crossbar.  Every step adds two partial sums that both partners
-  // hold, so all 16 lanes end with the same bits.
   v += dpp_move<0xB1>(v);
   v += dpp_move<0x4E>(v);
   v += dpp_move<0x141>(v);
@@ -52,21 +52,94 @@ __device__ inline double row_sum(double v) {
   return v;
 }
 
-// lanes per vector pair: one 16-lane DPP row (a wave works on four pairs at once)
-constexpr int kPairLanes = 16;
+// lanes per vector pair: one 16-lane DPP row (a wave works on four pairs at once; fewer
+// lanes per pair, with longer per-lane chains, measured slower)
+constexpr int P = 16;
 // global loads a thread keeps in flight while staging a block pair into LDS
 constexpr int kStage = 8;
 
+// Numerically null vectors.  With null_scale > 0 (a truncating caller), a vector whose
+// squared norm is at or below floor2 = null_scale k^2 |A|_F^2 (null_scale = eps^2: the
+// floor is k eps |A|_F) is numerically null: its direction is rounding noise, and the
+// overlap of such a pair is re-created by rounding every sweep, so it never stops
+// rotating.  Those pairs still rotate (the null vectors' norms keep converging to their
+// singular values) but do not count towards convergence: the sweeps stop once every pair
+// of vectors above the floor is orthogonal.  The length-L factor rows of null vectors are
+// then not orthonormal, so they are left zero unless the caller asks for their
+// completion.  null_scale = 0 (an exact SVD): every rotation counts, as in plain
+// one-sided Jacobi.
+__device__ inline double null_floor2(const double* fro2, const int64_t* pw, double null_scale) {
+  const double k = static_cast<double>(pw[0]);
+  return fro2[pw[10]] * (null_scale * k * k);
+}
+
+// |A|_F^2 of every problem (one workgroup each, fixed summation order).
+__global__ __launch_bounds__(256) void jacobi_norms(const int64_t* __restrict__ probs,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ fro2) {
+  __shared__ double part[4];
+  const int64_t* pw = probs + TN_SVD_PROB_WORDS * blockIdx.x;
+  const int64_t n = pw[0] * pw[1];
+  const double* xp = x + pw[2];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc = fma(xp[i], xp[i], acc);
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) fro2[pw[10]] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+// LDS elements a lane of a pair group moves per step of its dot products and rotations
+constexpr int kChunk = 8;
+
+// x_a <- x_a + ((c - 1) x_a - s x_b), x_b <- x_b + ((c - 1) x_b + s x_a) over n elements,
+// lane gl of the pair's P-lane group taking i = gl + P j.  Each chunk's loads are issued
+// together; u0/v0 (when given) already hold the first chunk.
+__device__ inline void rotate_rows(double* xa, double* xb, int n, int gl, double cm1, double s,
+                                   const double* u0, const double* v0) {
+  for (int base = 0; base < n; base += P * kChunk) {
+    double u[kChunk], v[kChunk];
+    if (base == 0 && u0 != nullptr) {
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        u[j] = u0[j];
+        v[j] = v0[j];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        const int i = base + gl + P * j;
+        const int ii = i < n ? i : 0;
+        u[j] = xa[ii];
+        v[j] = xb[ii];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const int i = base + gl + P * j;
+      if (i < n) {
+        xa[i] = u[j] + fma(cm1, u[j], -s * v[j]);
+        xb[i] = v[j] + fma(cm1, v[j], s * u[j]);
+      }
+    }
+  }
+}
+
 // circle method over n (even) players: round r in [0, n-1), pair p in [0, n/2)
 __device__ inline void pair_of(int n, int r, int p, int& a, int& b) {
-  a = p == 0 ? 0 : ((p - 1 + r) % (n - 1)) + 1;
-  b = ((n - 2 - p + r) % (n - 1)) + 1;
+  // both sums lie in [0, 2 (n - 1)): one conditional subtraction replaces the modulo
+  int ia = p - 1 + r, ib = n - 2 - p + r;
+  ia -= ia >= n - 1 ? n - 1 : 0;
+  ib -= ib >= n - 1 ? n - 1 : 0;
+  a = p == 0 ? 0 : ia + 1;
+  b = ib + 1;
 }
 
 __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* __restrict__ wgs,
                              double* __restrict__ x, double* __restrict__ q,
                              int32_t* __restrict__ rot_out, int w, int round, int first,
-                             double tol, const int32_t* __restrict__ stop) {
+                             double tol, const int32_t* __restrict__ stop,
+                             const double* __restrict__ fro2, double null_scale) {
   extern __shared__ double lds[];
   __shared__ int rot_s;
   // the launch's problems have converged (set by jacobi_converged after an earlier sweep):
@@ -144,10 +217,11 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
     }
   }
   // one pair per 16-lane group: a round's pairs run side by side
-  const int gl = threadIdx.x % kPairLanes;
-  const int grp = threadIdx.x / kPairLanes;
-  const int ngrp = blockDim.x / kPairLanes;
+  const int gl = threadIdx.x % P;
+  const int grp = threadIdx.x / P;
+  const int ngrp = blockDim.x / P;
   const double tol2 = tol * tol;
+  const double floor2 = null_floor2(fro2, pw, null_scale);
   int last = 0;
   for (int it = 0; it < inner_sweeps; ++it) {
     if (threadIdx.x == 0) rot_s = 0;
@@ -159,12 +233,32 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
         pair_of(ne, r, p, a, b);
         double* xa = xs + a * L;
         double* xb = xs + b * L;
+        // a lane's elements i = gl + P j, taken kChunk at a time with every LDS load of a
+        // chunk issued before its arithmetic (a lane past the end reads element 0 and
+        // contributes zero); the first chunk stays in registers for the rotation
         double al = 0.0, be = 0.0, ga = 0.0;
-        for (int i = gl; i < L; i += kPairLanes) {
-          const double u = xa[i], v = xb[i];
-          al = fma(u, u, al);
-          be = fma(v, v, be);
-          ga = fma(u, v, ga);
+        double u0[kChunk], v0[kChunk];
+        for (int base = 0; base < L; base += P * kChunk) {
+          double u[kChunk], v[kChunk];
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) {
+            const int i = base + gl + P * j;
+            const int ii = i < L ? i : 0;
+            u[j] = xa[ii];
+            v[j] = xb[ii];
+          }
+#pragma unroll
+          for (int j = 0; j < kChunk; ++j) {
+            const bool ok = base + gl + P * j < L;
+            const double uu = ok ? u[j] : 0.0, vv = ok ? v[j] : 0.0;
+            al = fma(uu, uu, al);
+            be = fma(vv, vv, be);
+            ga = fma(uu, vv, ga);
+            if (base == 0) {
+              u0[j] = uu;
+              v0[j] = vv;
+            }
+          }
         }
         al = row_sum(al);
         be = row_sum(be);
@@ -185,19 +279,9 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
           const double qd = 1.0 / (rr * den);
           const double s = t * den * qd;
           const double cm1 = -tt * rr * qd;
-          for (int i = gl; i < L; i += kPairLanes) {
-            const double u = xa[i], v = xb[i];
-            xa[i] = u + fma(cm1, u, -s * v);
-            xb[i] = v + fma(cm1, v, s * u);
-          }
-          double* qa = qs + a * k;
-          double* qb = qs + b * k;
-          for (int i = gl; i < k; i += kPairLanes) {
-            const double u = qa[i], v = qb[i];
-            qa[i] = u + fma(cm1, u, -s * v);
-            qb[i] = v + fma(cm1, v, s * u);
-          }
-          ++mine;
+          rotate_rows(xa, xb, L, gl, cm1, s, u0, v0);
+          rotate_rows(qs + a * k, qs + b * k, k, gl, cm1, s, nullptr, nullptr);
+          mine += al > floor2 && be > floor2;
         }
       }
       __syncthreads();
@@ -240,7 +324,8 @@ __global__ __launch_bounds__(256) void jacobi_converged(const int32_t* __restric
 // counting, normalized rows and Q rows scattered into U / S / Vh.
 __global__ void jacobi_finish(const int64_t* __restrict__ probs, const double* __restrict__ x,
                               const double* __restrict__ q, double* __restrict__ U,
-                              double* __restrict__ S, double* __restrict__ Vh) {
+                              double* __restrict__ S, double* __restrict__ Vh,
+                              const double* __restrict__ fro2, double null_scale, int complete) {
   extern __shared__ double sig[];
   int* rank = reinterpret_cast<int*>(sig + TN_SVD_MAX_K);
   const int64_t* pw = probs + TN_SVD_PROB_WORDS * blockIdx.x;
@@ -250,6 +335,8 @@ __global__ void jacobi_finish(const int64_t* __restrict__ probs, const double* _
   const int cols = static_cast<int>(pw[5]);
   const int64_t uo = pw[6], so = pw[7], vo = pw[8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // rows at or below the numerical-null floor carry no direction of their own
+  const double null_sigma = fmax(kNullSigma, sqrt(null_floor2(fro2, pw, null_scale)));
   for (int i = wave; i < k; i += nw) {
     const double* xr = x + xo + static_cast<int64_t>(i) * L;
     double acc = 0.0;
@@ -269,7 +356,7 @@ __global__ void jacobi_finish(const int64_t* __restrict__ probs, const double* _
   for (int i = 0; i < k; ++i) {
     const int rk = rank[i];
     const double si = sig[i];
-    const double inv = si > kNullSigma ? 1.0 / si : 0.0;
+    const double inv = si > null_sigma ? 1.0 / si : 0.0;
     const double* xr = x + xo + static_cast<int64_t>(i) * L;
     const double* qr = q + qo + static_cast<int64_t>(i) * k;
     if (!cols) {
@@ -290,8 +377,8 @@ __global__ void jacobi_finish(const int64_t* __restrict__ probs, const double* _
   // Vh[r, j] (rows <= cols) or U[j, r] (rows > cols).
   __syncthreads();
   int live = 0;
-  for (int i = 0; i < k; ++i) live += sig[i] > kNullSigma;
-  if (live == k) return;
+  for (int i = 0; i < k; ++i) live += sig[i] > null_sigma;
+  if (live == k || !complete) return;
   const int64_t wrow = cols ? 1 : L, wcol = cols ? k : 1;
   double* Wb = cols ? U + uo : Vh + vo;
   double* coef = sig;  // sigma is not needed past this point
@@ -360,10 +447,19 @@ extern "C" int tn_svd_jacobi_converged(const int32_t* rotations, int64_t n, int3
   return tn::check_launch("jacobi_converged");
 }
 
+extern "C" int tn_svd_jacobi_norms(const int64_t* probs, int64_t nprobs, const double* x, double* fro2,
+                                   void* stream) {
+  if (nprobs <= 0) return 0;
+  if (nprobs > 0x7fffffff) return tn::fail("tn_svd_jacobi_norms: too many problems");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  jacobi_norms<<<dim3(static_cast<unsigned>(nprobs)), 256, 0, s>>>(probs, x, fro2);
+  return tn::check_launch("jacobi_norms");
+}
+
 extern "C" int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int64_t nwgs,
                                    int64_t span, double* x, double* q, int32_t* rotations, int w,
                                    int round, int first, double tol, const int32_t* stop,
-                                   void* stream) {
+                                   const double* fro2, double null_scale, void* stream) {
   if (nwgs <= 0) return 0;
   if (w < 1 || w > TN_SVD_MAX_W) return tn::fail("tn_svd_jacobi_round: block width out of range");
   if (nwgs > 0x7fffffff) return tn::fail("tn_svd_jacobi_round: too many workgroups");
@@ -371,24 +467,27 @@ extern "C" int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int
   // and the (k + k % 2) (L + k) of a problem held by one block pair
   const size_t bytes = static_cast<size_t>(2 * w) * static_cast<size_t>(span) * sizeof(double);
   if (span <= 0) return tn::fail("tn_svd_jacobi_round: empty launch span");
+  if (fro2 == nullptr) return tn::fail("tn_svd_jacobi_round: no problem norms");
   if (int rc = allow_lds(reinterpret_cast<const void*>(jacobi_round), bytes, "tn_svd_jacobi_round"))
     return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int threads = w <= 4 ? 64 : (w >= 64 ? 1024 : 16 * w);  // one 16-lane group per pair
+  const int threads = w <= 4 ? 64 : (w >= 64 ? 1024 : P * w);  // one 16-lane group per pair
   hipLaunchKernelGGL(jacobi_round, dim3(static_cast<unsigned>(nwgs)), dim3(threads), bytes, s, probs,
-                     wgs, x, q, rotations, w, round, first, tol, stop);
+                     wgs, x, q, rotations, w, round, first, tol, stop, fro2, null_scale);
   return tn::check_launch("jacobi_round");
 }
 
 extern "C" int tn_svd_jacobi_finish(const int64_t* probs, int64_t nprobs, const double* x,
                                     const double* q, double* U, double* S, double* Vh,
+                                    const double* fro2, double null_scale, int complete,
                                     void* stream) {
   if (nprobs <= 0) return 0;
+  if (fro2 == nullptr) return tn::fail("tn_svd_jacobi_finish: no problem norms");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const size_t bytes = TN_SVD_MAX_K * (sizeof(double) + sizeof(int));
   if (int rc = allow_lds(reinterpret_cast<const void*>(jacobi_finish), bytes, "tn_svd_jacobi_finish"))
     return rc;
   hipLaunchKernelGGL(jacobi_finish, dim3(static_cast<unsigned>(nprobs)), dim3(1024), bytes, s, probs,
-                     x, q, U, S, Vh);
+                     x, q, U, S, Vh, fro2, null_scale, complete);
   return tn::check_launch("jacobi_finish");
 }

@@ -23,7 +23,7 @@ PROB_WORDS = 8
 SEG_WORDS = 6
 MPO_JOB_WORDS = 8
 MPO_TILE = 1024  # positions per tn_mpo_apply tile (256 threads x 4)
-SVD_PROB_WORDS = 10
+SVD_PROB_WORDS = 11
 SVD_MAX_W = 64
 SVD_MAX_K = 4096
 
@@ -51,9 +51,10 @@ _SIGNATURES = {
     "tn_scale": (_i32, [_i32, _i64, _d, _p, _i32, _p, _p]),
     "tn_cast": (_i32, [_i32, _p, _i32, _p, _i64, _p]),
     "tn_mpo_apply": (_i32, [_i32, _p, _p, _p, _i64, _p, _p, _p, _p, _i64, _p]),
-    "tn_svd_jacobi_round": (_i32, [_p, _p, _i64, _i64, _p, _p, _p, _i32, _i32, _i32, _d, _p, _p]),
+    "tn_svd_jacobi_norms": (_i32, [_p, _i64, _p, _p, _p]),
+    "tn_svd_jacobi_round": (_i32, [_p, _p, _i64, _i64, _p, _p, _p, _i32, _i32, _i32, _d, _p, _p, _d, _p]),
     "tn_svd_jacobi_converged": (_i32, [_p, _i64, _p, _p]),
-    "tn_svd_jacobi_finish": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p]),
+    "tn_svd_jacobi_finish": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p, _d, _i32, _p]),
     "tn_tridiag_ground": (_i32, [_p, _p, _i32, _d, _p, _p, _p]),
     "tn_tridiag_ground_host": (_i32, [_p, _p, _i32, _d, _p, _p]),
     "tn_tridiag_ground_batched": (_i32, [_p, _p, _i32, _i64, _i64, _d, _p, _p, _p]),

@@ -317,7 +317,7 @@ class _JacobiPlan:
                 nb = max(2, -(-k // w))
                 nb += nb % 2
             inner = _SVD_SINGLE_SWEEPS if nb == 2 else _SVD_INNER_SWEEPS
-            rows.append([k, L, xo, qo, nb, int(cols), uo, so, vo, inner])
+            rows.append([k, L, xo, qo, nb, int(cols), uo, so, vo, inner, len(rows)])
             widths.append(w)
             self.slices[c] = (uo, m, k, so, vo, n)
             xo += k * L
@@ -368,9 +368,22 @@ def _jacobi_plan(ml: _MatrixLayout) -> _JacobiPlan:
     return plan
 
 
-def _svd_parts(packed) -> dict:
+_EPS = 2.220446049250313e-16
+
+
+def _svd_parts(packed, complete: bool = True) -> dict:
     """FP64 thin SVD of every packed sector matrix with the batched Jacobi kernels:
-    ``{c: (U, S, Vh)}``, singular values descending, as device views."""
+    ``{c: (U, S, Vh)}``, singular values descending, as device views.  ``complete=True``:
+    plain one-sided Jacobi to convergence, U and Vh isometries on every column.
+    ``complete=False`` (a truncating caller): the sweeps stop once the vectors above the
+    numerical-null floor ``k eps |A|_F`` are orthogonal, and the factor rows of the values
+    at or below it stay zero (such values are never kept by a cutoff)."""
+    return _svd_run(packed, complete)[0]
+
+
+def _svd_run(packed, complete: bool):
+    """``_svd_parts`` and a function that completes the null rows afterwards (in place,
+    the returned views see it), for a caller that decides on them later."""
     from tenet import _native
 
     buf, ml = packed
@@ -384,6 +397,13 @@ def _svd_parts(packed) -> dict:
     q = torch.empty(plan.q_size, dtype=torch.float64, device=dev)
     if plan.gather is not None:
         plan.gather.run(buf, x)
+    # a truncating caller stops the sweeps once the vectors above k eps |A|_F are
+    # orthogonal; an exact SVD runs plain one-sided Jacobi
+    null_scale = 0.0 if complete else _EPS * _EPS
+    fro2 = torch.empty(max(plan.nprobs, 1), dtype=torch.float64, device=dev)
+    if plan.nprobs:
+        _native.check(lib.tn_svd_jacobi_norms(plan.probs.get(dev)["probs"].data_ptr(), plan.nprobs,
+                                              x.data_ptr(), fro2.data_ptr(), stream))
     rots = []
     for g in plan.groups:
         g["tb"] = g["tables"].get(dev)
@@ -411,7 +431,8 @@ def _svd_parts(packed) -> dict:
                     _native.check(lib.tn_svd_jacobi_round(
                         g["tb"]["probs"].data_ptr(), g["tb"]["wgs"].data_ptr(), g["nwgs"], g["span"],
                         x.data_ptr(), q.data_ptr(), rot[r * g["nwgs"]:].data_ptr(), g["w"], r,
-                        int(sweep == 0 and r == 0), g["tol"], flag.data_ptr(), st))
+                        int(sweep == 0 and r == 0), g["tol"], flag.data_ptr(), fro2.data_ptr(), null_scale,
+                        st))
                 _native.check(lib.tn_svd_jacobi_converged(rot.data_ptr(), g["rounds"] * g["nwgs"],
                                                           flag.data_ptr(), st))
         done += batch
@@ -425,10 +446,15 @@ def _svd_parts(packed) -> dict:
     U = torch.empty(plan.u_size, dtype=torch.float64, device=dev)
     S = torch.empty(plan.s_size, dtype=torch.float64, device=dev)
     Vh = torch.empty(plan.v_size, dtype=torch.float64, device=dev)
-    if plan.nprobs:
-        _native.check(lib.tn_svd_jacobi_finish(
-            plan.probs.get(dev)["probs"].data_ptr(), plan.nprobs, x.data_ptr(), q.data_ptr(),
-            U.data_ptr(), S.data_ptr(), Vh.data_ptr(), stream))
+
+    def finish(full: int) -> None:
+        if plan.nprobs:
+            _native.check(lib.tn_svd_jacobi_finish(
+                plan.probs.get(dev)["probs"].data_ptr(), plan.nprobs, x.data_ptr(), q.data_ptr(),
+                U.data_ptr(), S.data_ptr(), Vh.data_ptr(), fro2.data_ptr(), null_scale, full,
+                _native.stream_ptr()))
+
+    finish(int(complete))
     parts = {}
     for c in ml.sectors:
         sl = plan.slices[c]
@@ -439,7 +465,21 @@ def _svd_parts(packed) -> dict:
             continue
         uo, m, k, so, vo, n = sl
         parts[c] = (U[uo : uo + m * k].view(m, k), S[so : so + k], Vh[vo : vo + k * n].view(k, n))
-    return parts
+    return parts, (lambda: None) if complete else (lambda: finish(1))
+
+
+def _needs_completion(values: dict, keep: dict) -> bool:
+    """Whether a kept singular value may lie at or below its sector's numerical-null floor
+    (``k eps |A|_F``, with ``|A|_F^2`` taken as the sum of the sector's squared values and
+    a factor two of margin against rounding), so its factor rows must be completed."""
+    for c, n in keep.items():
+        v = values.get(c)
+        if v is None or n <= 0:
+            continue
+        floor = 2.0 * len(v) * _EPS * float(np.sqrt(np.sum(np.square(v, dtype=np.float64))))
+        if float(v[n - 1]) <= max(floor, 1e-290):
+            return True
+    return False
 
 
 def _factors(m: SymmetricTensor, parts: dict, keep: dict, scale: float = 1.0):
@@ -506,11 +546,11 @@ def select_bond(t: SymmetricTensor, axes=None, *, max_bond=None, cutoff=None, cu
     """The truncation decision alone (reference ``linalg.py:1327-1438``)."""
     _validate(max_bond, cutoff, cutoff_mode, renorm, "select_bond")
     m, _, packed = _lower(t, axes)
-    parts = _svd_parts(packed)
+    parts = _svd_parts(packed, complete=False)  # the decision reads the values only
     return _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, "select_bond")
 
 
-def _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, caller):
+def _host_values(parts, caller) -> dict:
     cs = list(parts)
     if not cs:
         raise ValueError(f"{caller}: this tensor has no singular values at all")
@@ -520,7 +560,25 @@ def _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, caller):
         n = parts[c][1].numel()
         values[c] = flat[o : o + n]
         o += n
+    return values
+
+
+def _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, caller, values=None):
+    values = _host_values(parts, caller) if values is None else values
     return _decide(_spectrum(values), m.provider, max_bond, cutoff, cutoff_mode, renorm, caller)
+
+
+def _truncated(t, axes, max_bond, cutoff, cutoff_mode, renorm):
+    """The truncating SVD: Jacobi without null-row completion, the keep rule on the host
+    values, and the completion only when a kept value is numerically null."""
+    m, _, packed = _lower(t, axes)
+    parts, complete = _svd_run(packed, False)
+    values = _host_values(parts, "svd_truncated")
+    sel = _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, "svd_truncated", values)
+    keep = dict(sel.bond.sectors)
+    if _needs_completion(values, keep):
+        complete()
+    return m, parts, keep, sel
 
 
 def svd_truncated(t: SymmetricTensor, axes=None, *, max_bond=None, cutoff=None,
@@ -531,20 +589,16 @@ def svd_truncated(t: SymmetricTensor, axes=None, *, max_bond=None, cutoff=None,
     ``(bond OUT, bond IN)``, ``Vh`` on ``(bond OUT, *right legs)``.
     """
     _validate(max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
-    m, _, packed = _lower(t, axes)
-    parts = _svd_parts(packed)
-    sel = _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
-    return _factors(m, parts, dict(sel.bond.sectors), sel.scale)
+    m, parts, keep, sel = _truncated(t, axes, max_bond, cutoff, cutoff_mode, renorm)
+    return _factors(m, parts, keep, sel.scale)
 
 
 def svd_truncated_with_selection(t, axes=None, *, max_bond=None, cutoff=None, cutoff_mode="rsum2",
                                  renorm=False):
     """``svd_truncated`` that also returns its ``BondSelection``."""
     _validate(max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
-    m, _, packed = _lower(t, axes)
-    parts = _svd_parts(packed)
-    sel = _select(m, parts, max_bond, cutoff, cutoff_mode, renorm, "svd_truncated")
-    return (*_factors(m, parts, dict(sel.bond.sectors), sel.scale), sel)
+    m, parts, keep, sel = _truncated(t, axes, max_bond, cutoff, cutoff_mode, renorm)
+    return (*_factors(m, parts, keep, sel.scale), sel)
 
 
 def lq(t: SymmetricTensor, axes=None):

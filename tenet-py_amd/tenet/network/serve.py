@@ -235,10 +235,11 @@ class DMRGServer:
         return sorted(done, key=lambda r: r.rid)
 
 
-def _gpu_server(max_active: int, precision: str, share: float) -> DMRGServer:
+def _gpu_server(max_active: int, precision: str, share: float, mode: str = "streams") -> DMRGServer:
     """A worker's server: ``share`` of the HBM free when it starts."""
     free, _ = torch.cuda.mem_get_info()
-    return DMRGServer(memory_budget=int(0.8 * free * share), max_active=max_active, precision=precision)
+    return DMRGServer(memory_budget=int(0.8 * free * share), max_active=max_active, precision=precision,
+                      mode=mode)
 
 
 def _worker_main(wid: int, req_q, res_q, factory) -> None:
@@ -304,11 +305,13 @@ class MultiProcessServer:
     ``DMRGServer`` worker interface) replaces the GPU server in host-only tests."""
 
     def __init__(self, *, workers: int = 4, max_active: int = 8, precision: str = "fp32",
-                 factory=None) -> None:
+                 factory=None, mode: str = "streams") -> None:
+        """``mode``: each worker's ``DMRGServer`` mode (``"shared"``: its active solves'
+        Lanczos steps run as one batched Krylov solve per tick)."""
         if workers < 1 or max_active < workers:
             raise ValueError(f"need 1 <= workers <= max_active, got {workers}, {max_active}")
         self.workers = workers
-        self.factory = factory or partial(_gpu_server, max_active // workers, precision, 1.0 / workers)
+        self.factory = factory or partial(_gpu_server, max_active // workers, precision, 1.0 / workers, mode)
         self.boot_s = None
 
     def serve(self, requests, *, boot_timeout: float = 600.0, on_result=None) -> list:

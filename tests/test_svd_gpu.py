@@ -100,3 +100,66 @@ def test_jacobi_svd_two_calls_share_nothing():
     a2, p2 = _run([(20, 30), (100, 90)], seed=5)
     for c in a1:
         assert torch.equal(p1[c][1], p2[c][1])
+
+
+def test_null_rows_are_completed_on_demand():
+    """A DMRG-like spectrum decaying into rounding noise, through the truncating callers'
+    form: the sweeps stop once the vectors above the numerical-null floor (k eps |A|_F) are
+    orthogonal and the length-L factor rows below it stay zero; every value matches LAPACK
+    (the null ones to the floor), the live factor rows are orthonormal and reproduce A's
+    live part, and the on-demand completion makes both factors isometries."""
+    from tenet import linalg
+
+    rng = np.random.default_rng(7)
+    shapes = [(96, 100), (130, 128), (40, 44)]
+    ml = _Layout(shapes)
+    mats, flat = {}, np.empty(ml.size)
+    for c, (m, n) in enumerate(shapes):
+        k = min(m, n)
+        u, _ = np.linalg.qr(rng.standard_normal((m, k)))
+        v, _ = np.linalg.qr(rng.standard_normal((n, k)))
+        a = (u * np.exp(-40.0 / k * np.arange(k))) @ v.T  # 1 down to ~1e-17
+        mats[c] = a
+        flat[ml.offsets[c] : ml.offsets[c] + m * n] = a.reshape(-1)
+    buf = torch.from_numpy(flat).cuda()
+    parts, complete = linalg._svd_run((buf, ml), False)
+    for c, a in mats.items():
+        u, s, vh = (t.cpu().numpy() for t in parts[c])
+        ref = np.linalg.svd(a, compute_uv=False)
+        assert np.abs(s - ref).max() <= 1e-13 * ref.max()
+        floor = min(a.shape) * 2.220446049250313e-16 * np.linalg.norm(a)
+        live = s > floor
+        assert 0 < live.sum() < len(s)  # the spectrum crosses the floor
+        w = vh if a.shape[0] <= a.shape[1] else u.T
+        assert np.all(w[~live] == 0.0)
+        assert np.abs(w[live] @ w[live].T - np.eye(live.sum())).max() < 1e-12
+        assert np.abs(u[:, live] * s[live] @ vh[live] - a).max() <= 1e-12 * ref.max()
+    complete()
+    _check(mats, parts)
+
+
+def test_truncating_form_keeps_the_values_of_the_exact_one():
+    """Graded and rank-deficient sectors through both forms: the same singular values to
+    FP64 rounding, and the live factor rows agree up to sign."""
+    from tenet import linalg
+
+    rng = np.random.default_rng(11)
+    shapes = [(48, 48), (96, 80), (33, 120), (200, 190)]
+    ml = _Layout(shapes)
+    flat = np.empty(ml.size)
+    for c, (m, n) in enumerate(shapes):
+        k = min(m, n)
+        u, _ = np.linalg.qr(rng.standard_normal((m, k)))
+        v, _ = np.linalg.qr(rng.standard_normal((n, k)))
+        s = 10.0 ** -np.linspace(0, 18, k)
+        s[k // 2 :: 5] = 0.0
+        flat[ml.offsets[c] : ml.offsets[c] + m * n] = ((u * s) @ v.T).reshape(-1)
+    buf = torch.from_numpy(flat).cuda()
+    exact = linalg._svd_parts((buf, ml), complete=True)
+    trunc = linalg._svd_parts((buf, ml), complete=False)
+    for c in exact:
+        se, st = exact[c][1].cpu().numpy(), trunc[c][1].cpu().numpy()
+        assert np.abs(se - st).max() <= 1e-13 * se.max()
+        live = st > 1e-6 * st.max()  # well separated values: vectors fixed up to sign
+        ue, ut = exact[c][0].cpu().numpy()[:, live], trunc[c][0].cpu().numpy()[:, live]
+        assert np.abs(np.abs(np.sum(ue * ut, axis=0)) - 1.0).max() < 1e-9
