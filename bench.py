@@ -721,7 +721,7 @@ def bench_serve(args) -> int:
                        "server": (f"{workers} worker processes x {8 // workers} slots"
                                   if arms[0][0].endswith("proc") else arms[0][0])},
             "p50_s": head["p50_s"], "p99_s": head["p99_s"], "max_dE_rel_vs_fp64_serial": de,
-            "arms": {k: {kk: vv for kk, vv in v.items() if kk != "energies"} for k, v in out.items()}}
+            "arms": out}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -25,7 +25,7 @@ from tenet import ops
 from tenet.engine import contraction_exec, inplace_map, recouple_program, require_device
 from tenet.plans import repartition_plan
 from tenet.space import OUT
-from tenet.structure import TensorStructure
+from tenet.structure import TensorStructure, _axis_index
 from tenet.tensor import SymmetricTensor
 
 __all__ = ["Heff2", "HEFF2_STEPS"]
@@ -347,13 +347,6 @@ class _FusedChain:
         return out
 
 
-def _sector_rows(s: TensorStructure) -> tuple:
-    """Per block of ``s`` (in block order) the space sector of every axis."""
-    from tenet.structure import _axis_sectors
-
-    return _axis_sectors(s)
-
-
 def _mpo_jobs(inputs, outputs, w12, src_size, dst_size):
     """``tn_mpo_apply`` tables from per-(a, s) channel lists.  ``inputs[(a, s)]`` and
     ``outputs[(a, s)]`` hold ``(offset, row_stride, dense_index, R, S)`` per channel slab;
@@ -368,9 +361,11 @@ def _mpo_jobs(inputs, outputs, w12, src_size, dst_size):
         if S <= 0 or R <= 0 or R * S >= 2**32:
             raise ValueError(f"two-site MPO slab pair of {R} x {S} positions is outside the kernel's range")
         cm = np.zeros((len(olist), 32), dtype=np.float64)
-        for oi, o in enumerate(olist):
-            for ki, i in enumerate(ilist):
-                cm[oi, ki] = w12[o[2] + i[2]]
+        if ilist:
+            od = np.array([o[2] for o in olist], dtype=np.int64)  # (x, P, Q) per output
+            idn = np.array([i[2] for i in ilist], dtype=np.int64)  # (p, q, y) per input
+            cm[:, :len(ilist)] = w12[od[:, 0, None], od[:, 1, None], od[:, 2, None],
+                                     idn[None, :, 0], idn[None, :, 1], idn[None, :, 2]]
         j = len(jobs)
         jobs.append((R, S, len(ins), len(ins) + len(ilist), len(outs), len(outs) + len(olist), len(coef), 0))
         ins.extend((i[0], i[1]) for i in ilist)
@@ -426,8 +421,24 @@ class _SectorChain:
         w2, o2 = w_right[0], w_right[1]  # (m, Q, q, y)
         w12 = np.einsum("xPpm,mQqy->xPQpqy", w1, w2)
         L = s_in.layout
+        # host tables as plain lists, and every block's sectors as positions in its legs'
+        # spaces (aa's a and B legs and FL's outer legs share the left bond space, aa's r
+        # leg and FR's outer legs the right one, so positions compare across tensors)
+        if not (s_in.legs[0].space == fl_s.legs[0].space == fl_s.legs[2].space
+                and s_in.legs[3].space == fr_s.legs[0].space == fr_s.legs[2].space):
+            raise ValueError("two-site H_eff: the environments' bond spaces do not match the vector's")
+        shp, offs = L.shapes.tolist(), L.offsets.tolist()
+        fr_shp, fr_offs = fr_s.layout.shapes.tolist(), fr_s.layout.offsets.tolist()
+        fl_shp, fl_offs = fl_s.layout.shapes.tolist(), fl_s.layout.offsets.tolist()
+        sec_in, sec_fr, sec_fl = (_axis_index(x).tolist() for x in (s_in, fr_s, fl_s))
+
+        def dense_of(leg, o):  # W's dense offset of every sector of a leg's space, by position
+            return [o.get(sec) for sec, _ in leg.space.sectors]
+
+        o_p, o_q = dense_of(s_in.legs[1], o1[2]), dense_of(s_in.legs[2], o2[2])
+        o_P, o_Q = dense_of(s_in.legs[1], o1[1]), dense_of(s_in.legs[2], o2[1])
+        o_y, o_x = dense_of(fr_s.legs[1], o2[3]), dense_of(fl_s.legs[1], o1[0])
         # ---- entry: aa blocks grouped by r ----
-        sec_in, sec_fr, sec_fl = (_sector_rows(x) for x in (s_in, fr_s, fl_s))
         rows_r: dict = {}
         for i, (a, p, q, r) in enumerate(sec_in):
             rows_r.setdefault(r, []).append(i)
@@ -440,40 +451,36 @@ class _SectorChain:
         t1_base, t1_acc = {}, 0
         problems1, problems1_rows, rows1 = [], [], []
         for r, blocks in rows_r.items():
-            dr = int(L.shapes[blocks[0]][3])
+            dr = shp[blocks[0]][3]
             xs_base[r] = acc
             m = 0
             for i in blocks:
-                da, dp, dq, _ = (int(v) for v in L.shapes[i])
+                da, dp, dq, _ = shp[i]
                 x_rowoff[i] = m
-                n = da * dp * dq * dr
-                xs_regions.append((acc + m * dr, (n,), (1,), (1,), [(int(L.offsets[i]), 1.0)]))
+                xs_regions.append((acc + m * dr, (da * dp * dq * dr,), (1,), (1,), [(offs[i], 1.0)]))
                 m += da * dp * dq
             acc += m * dr
             fcols = cols_r.get(r, [])
             if not fcols:
                 continue
-            if r not in frs_base:
-                frs_base[r] = facc
-                c0 = 0
-                for i in fcols:
-                    fdr, my, ds = (int(v) for v in fr_s.layout.shapes[i])
-                    fr_coloff[i] = c0
-                    c0 += my * ds
-                n_r[r] = c0
-                for i in fcols:
-                    fdr, my, ds = (int(v) for v in fr_s.layout.shapes[i])
-                    frs_regions.append((facc + fr_coloff[i], (fdr, my * ds), (c0, 1), (my * ds, 1),
-                                        [(int(fr_s.layout.offsets[i]), 1.0)]))
-                facc += dr * c0
+            frs_base[r] = facc
+            c0 = 0
+            for i in fcols:
+                fr_coloff[i] = c0
+                c0 += fr_shp[i][1] * fr_shp[i][2]
+            n_r[r] = c0
+            for i in fcols:
+                fdr, my, ds = fr_shp[i]
+                frs_regions.append((facc + fr_coloff[i], (fdr, my * ds), (c0, 1), (my * ds, 1),
+                                    [(fr_offs[i], 1.0)]))
+            facc += dr * c0
             t1_base[r] = t1_acc
-            problems1.append((m, n_r[r], t1_acc, n_r[r], [(xs_base[r], frs_base[r], dr, dr, n_r[r])]))
+            problems1.append((m, c0, t1_acc, c0, [(xs_base[r], frs_base[r], dr, dr, c0)]))
             # the same problem reading x_r's rows where they lie in aa's blocks
-            problems1_rows.append((m, n_r[r], t1_acc, n_r[r], [(0, frs_base[r], dr, dr, n_r[r])]))
-            rows1.append(np.concatenate([int(L.offsets[i]) + dr * np.arange(int(np.prod(L.shapes[i][:3])),
-                                                                             dtype=np.int64)
-                                         for i in blocks]))
-            t1_acc += m * n_r[r]
+            problems1_rows.append((m, c0, t1_acc, c0, [(0, frs_base[r], dr, dr, c0)]))
+            rows1.append(np.concatenate([offs[i] + dr * np.arange(shp[i][0] * shp[i][1] * shp[i][2],
+                                                                  dtype=np.int64) for i in blocks]))
+            t1_acc += m * c0
         self.xs_size, self.frs_size, self.t1_size = acc, facc, t1_acc
         # host specs kept beside the compiled programs: BatchedHeff2 merges them across bonds
         self.xs_regions, self.frs_regions, self.problems1 = xs_regions, frs_regions, problems1
@@ -497,21 +504,18 @@ class _SectorChain:
         t3_base, out_coloff, n_b, t3_acc = {}, {}, {}, 0
         problems3, order3 = [], []
         for B, outs_j in out_cols.items():
-            db = int(L.shapes[outs_j[0]][0])
+            db = shp[outs_j[0]][0]
             c0 = 0
             for j in outs_j:
-                _, dP, dQ, ds = (int(v) for v in L.shapes[j])
                 out_coloff[j] = c0
-                c0 += dP * dQ * ds
+                c0 += shp[j][1] * shp[j][2] * shp[j][3]
             n_b[B] = c0
-            frows = fl_rows.get(B, [])
             kb = 0
             fls_base[B] = lacc
-            for i in frows:
-                da, mx, fdb = (int(v) for v in fl_s.layout.shapes[i])
+            for i in fl_rows.get(B, ()):
+                da, mx, fdb = fl_shp[i]
                 fl_rowoff[i] = kb
-                fls_regions.append((lacc + kb * fdb, (da * mx * fdb,), (1,), (1,),
-                                    [(int(fl_s.layout.offsets[i]), 1.0)]))
+                fls_regions.append((lacc + kb * fdb, (da * mx * fdb,), (1,), (1,), [(fl_offs[i], 1.0)]))
                 kb += da * mx
             k_b[B] = kb
             lacc += kb * db
@@ -533,31 +537,33 @@ class _SectorChain:
         for i, (a, p, q, r) in enumerate(sec_in):
             if r not in t1_base:
                 continue
-            da, dp, dq, _ = (int(v) for v in L.shapes[i])
+            da, dp, dq, _ = shp[i]
             N = n_r[r]
+            row0 = t1_base[r] + x_rowoff[i] * N
             for fi, y, sg in fr_by_r_s.get(r, ()):
-                _, my, ds = (int(v) for v in fr_s.layout.shapes[fi])
+                my, ds = fr_shp[fi][1], fr_shp[fi][2]
+                lst = inputs.setdefault((a, sg), [])
                 for ip in range(dp):
                     for iq in range(dq):
                         for iy in range(my):
-                            off = (t1_base[r] + (x_rowoff[i] + ip * dq + iq) * N + fr_coloff[fi] + iy * ds)
-                            dense = (o1[2][p] + ip, o2[2][q] + iq, o2[3][y] + iy)
-                            inputs.setdefault((a, sg), []).append((off, dp * dq * N, dense, da, ds))
+                            lst.append((row0 + (ip * dq + iq) * N + fr_coloff[fi] + iy * ds, dp * dq * N,
+                                        (o_p[p] + ip, o_q[q] + iq, o_y[y] + iy), da, ds))
         outputs: dict = {}
         for B in order3:
             N = n_b[B]
             for fi in fl_rows.get(B, ()):
                 a, x, _ = sec_fl[fi]
-                da, mx, _ = (int(v) for v in fl_s.layout.shapes[fi])
+                da, mx = fl_shp[fi][0], fl_shp[fi][1]
                 for j in out_cols[B]:
-                    _, P, Q, sg = sec_in[j]
-                    _, dP, dQ, ds = (int(v) for v in L.shapes[j])
+                    P, Q, sg = sec_in[j][1], sec_in[j][2], sec_in[j][3]
+                    dP, dQ, ds = shp[j][1], shp[j][2], shp[j][3]
+                    lst = outputs.setdefault((a, sg), [])
                     for ix in range(mx):
+                        row = t3_base[B] + (fl_rowoff[fi] + ix) * N + out_coloff[j]
                         for iP in range(dP):
                             for iQ in range(dQ):
-                                off = (t3_base[B] + (fl_rowoff[fi] + ix) * N + out_coloff[j] + (iP * dQ + iQ) * ds)
-                                dense = (o1[0][x] + ix, o1[1][P] + iP, o2[1][Q] + iQ)
-                                outputs.setdefault((a, sg), []).append((off, mx * N, dense, da, ds))
+                                lst.append((row + (iP * dQ + iQ) * ds, mx * N,
+                                            (o_x[x] + ix, o_P[P] + iP, o_Q[Q] + iQ), da, ds))
         jobs, ins, outs, coef, tiles = _mpo_jobs(inputs, outputs, w12, max(t1_acc, 1), max(t3_acc, 1))
         self.mpo_host = (jobs, ins, outs, coef)
         self.mpo_njobs = len(jobs)
@@ -577,9 +583,9 @@ class _SectorChain:
         for p, B in enumerate(order3):
             N = n_b[B]
             for j in out_cols[B]:
-                db, dP, dQ, ds = (int(v) for v in L.shapes[j])
+                db, dP, dQ, ds = shp[j]
                 n = dP * dQ * ds
-                self.exit_spec.append((int(L.offsets[j]), (db, n), (n, 1), (N, 1), p, out_coloff[j]))
+                self.exit_spec.append((offs[j], (db, n), (n, 1), (N, 1), p, out_coloff[j]))
         self.exit = RecoupleProgram(exit_regions(self.exit_spec, self.gemm4.slices),
                                     max(self.gemm4.ws_size, 1), s_in.size)
         self.flops = self.gemm1.flops + self.gemm4.flops
