@@ -40,7 +40,7 @@ PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6, "bf16x3": 2500.0}
 # scaled by what the same counters read for `bench.py --workload copy` (a 1 GiB device
 # copy: FETCH_SIZE 524302 KiB for 1048576 KiB read, WRITE_SIZE 1048576 KiB written).
 PMC_TRAFFIC = {
-    (64, 2048, "fp32"): {"fetch_kib": {0: 28252.6, 3: 37311.1}, "write_kib": {0: 30295.2, 3: 23977.9},
+    (64, 2048, "fp32"): {"fetch_kib": {0: 32795.0, 3: 37416.6}, "write_kib": {0: 30349.4, 3: 24009.3},
                          "fetch_scale": 0.5, "write_scale": 1.0,
                          "source": "profiles/heff2_L64_D2048_fp32_traffic.md"},
 }
@@ -253,13 +253,14 @@ def bench_heff2(args) -> int:
                       for k in big) * 1024.0
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
                 "frac": achieved / PEAK_TFLOPS[head], "traffic": traffic,
-                "kernel": f"gemm_kernel (tn_gemm_grouped, {head}): the two H_eff GEMM launches of one "
+                "kernel": f"gemm_pipe_kernel ({head}; step 1 through tn_gemm_grouped_rows, step 3 through "
+                          "tn_gemm_grouped): the two H_eff GEMM launches of one "
                           f"matvec, steps {sorted(big)}; achieved = their flops / their summed average "
                           "kernel durations, traffic = their summed HBM bytes",
                 "avg_launch_ms": {str(k): per_step[k][0] for k in per_step},
                 "flops_per_launch": {str(k): per_step[k][1] for k in per_step},
-                # the passes around the GEMMs: entry (block gather + cast), exit (split-K
-                # slice sum + scatter into the vector's blocks + cast), plain reductions
+                # the passes around the GEMMs: exit (split-K slice sum + scatter into the
+                # vector's blocks + cast; the FP32 step 1 reads the vector's blocks itself)
                 "aux_passes_avg_ms": reduce_ms,
                 "traffic_source": pmc["source"] if pmc else None}
 
@@ -273,7 +274,7 @@ def bench_heff2(args) -> int:
                          "frac": 3 * eq / PEAK_TFLOPS["bf16x3"],
                          "fp32_equivalent_tflops": eq, "fp32_equivalent_frac_of_fp32_peak": eq / PEAK_TFLOPS["fp32"],
                          "avg_launch_ms": {str(k): b_steps[k][0] for k in b_big},
-                         "kernel": "gemm_kernel (tn_gemm_grouped, bf16x3): the two H_eff GEMM launches; "
+                         "kernel": "gemm_pipe_kernel (bf16x3): the two H_eff GEMM launches; "
                                    "achieved counts 3 BF16 products per FP32 flop"}
 
     # the headline arm's matvec captured as a hipGraph and replayed on resident buffers
@@ -696,7 +697,7 @@ def bench_serve(args) -> int:
         srv = make()
         t0 = time.perf_counter()
         res = srv.serve(reqs, on_result=lambda r, n=name: log(
-            f"{n}: request {r.rid} done at {r.finished:.1f}s ({r.sweeps} sweeps)"))
+            f"{n}: request {r.rid} done at {r.finished:.1f}s ({r.sweeps} sweeps, E={r.energy:.15f})"))
         wall = time.perf_counter() - t0
         boot = getattr(srv, "boot_s", None)
         rep = latency_report(res, wall - (boot or 0.0))
