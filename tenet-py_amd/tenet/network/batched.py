@@ -214,12 +214,16 @@ class BatchedHeff2:
         self.fr = torch.cat([a for a, _ in stacks])
         self.fl = torch.cat([b for _, b in stacks])
         entry, p1, p3 = [], [], []
+        p1_rows, rows1 = [], []
         jobs, ins, outs, coef, tl = [], [], [], [], []
         for i, ch in enumerate(chains):
             xo, so = int(self.x_off[i]), int(xs_off[i])
             entry += [(d + so, shp, ds, ss, [(t + xo, c) for t, c in terms])
                       for d, shp, ds, ss, terms in ch.xs_regions]
             p1 += _shift_problems(ch.problems1, int(t1_off[i]), so, int(frs_off[i]))
+            # the row-gathered form: every instance's step-1 rows read in place in the arena
+            p1_rows += _shift_problems(ch.gemm1_rows.problems, int(t1_off[i]), 0, int(frs_off[i]))
+            rows1 += [r + xo for r in ch.gemm1_rows.a_rows]
             p3 += _shift_problems(ch.problems3, 0, int(fls_off[i]), int(t3_off[i]))
             i0, o0, c0 = len(ins), len(outs), len(coef)
             cj, ci, co, cc = ch.mpo_host
@@ -233,6 +237,10 @@ class BatchedHeff2:
         self.entry = RecoupleProgram(entry, self.x_size, max(self.xs_size, 1))
         self.gemm1 = GemmProgram(p1, max(self.xs_size, 1), int(frs_off[-1]) or 1, max(self.t1_size, 1),
                                  tile=tune.get("gemm1") or (64, 64))
+        t1_tile = tune.get("gemm1") or (64, 64)
+        self.gemm1_rows = GemmProgram(p1_rows, max(self.x_size, 1), int(frs_off[-1]) or 1, max(self.t1_size, 1),
+                                      tile=t1_tile if t1_tile in GemmProgram.ROWS_TILES else (64, 64),
+                                      a_rows=rows1)
         self.gemm4 = GemmProgram(p3, int(fls_off[-1]) or 1, max(self.t3_size, 1), 1, trans_a=True,
                                  tile=tune.get("gemm4") or (64, 64), gather=True)
         ex = []
@@ -281,10 +289,15 @@ class BatchedHeff2:
     def _apply_sector(self, x: torch.Tensor, out_dtype: torch.dtype, tiles: dict | None = None) -> torch.Tensor:
         t = tiles or {}
         wt = self.dtype
-        xs = torch.empty(max(self.xs_size, 1), dtype=wt, device=x.device)
-        self.entry.run(x, xs)
         t1 = torch.empty(max(self.t1_size, 1), dtype=wt, device=x.device)
-        self.gemm1.run(xs, self.fr, t1, self.precision, tile=t.get("gemm1"))
+        tile1 = tuple(t["gemm1"]) if t.get("gemm1") else self.gemm1_rows.tile
+        if wt == torch.float32 and tile1 in GemmProgram.ROWS_TILES:
+            # FP32 / BF16x3: step 1 reads the arena's blocks in place (FP64 or FP32), no entry pass
+            self.gemm1_rows.run(x, self.fr, t1, self.precision, tile=tile1)
+        else:
+            xs = torch.empty(max(self.xs_size, 1), dtype=wt, device=x.device)
+            self.entry.run(x, xs)
+            self.gemm1.run(xs, self.fr, t1, self.precision, tile=t.get("gemm1"))
         t3 = torch.empty(max(self.t3_size, 1), dtype=wt, device=x.device)
         self._mpo(t1, t3)
         ws = torch.empty(max(self.gemm4.ws_size, 1), dtype=wt, device=x.device)
