@@ -8,12 +8,15 @@
 //
 // Vectors are grouped in blocks of w.  One launch is one round of a circle-method
 // tournament over the blocks of every problem: each workgroup takes one block pair, stages
-// its 2w vectors and their Q rows in LDS, runs inner tournaments over the 2w vectors
-// (every pair once per inner sweep; the pairs of a round side by side, one 16-lane group
-// each, at most 64 groups), and writes both back.
+// its 2w vectors and their Q rows in LDS, rotates the pairs it is due (round 0 of a sweep:
+// every pair of the 2w vectors; later rounds: the w x w pairs across the two blocks; the
+// pairs of an inner round side by side, one 16-lane group each, at most 64 groups), and
+// writes both back.
 // A problem whose vectors fit in one block pair converges inside a single launch.
 // Each workgroup reports the rotations of its last inner sweep, so the host stops when a
 // whole sweep reports none.
+
+#include <algorithm>
 
 #include "tn_common.hpp"
 
@@ -53,8 +56,27 @@ __device__ inline double row_sum(double v) {
 }
 
 // lanes per vector pair: one 16-lane DPP row (a wave works on four pairs at once; fewer
-// lanes per pair, with longer per-lane chains, measured slower)
+// lanes per pair, with longer per-lane chains, measured slower), or two rows for the long
+// vectors of block tournaments (jacobi_round's PL)
 constexpr int P = 16;
+
+// the sum over a pair's PL lanes, the same bits in every lane: the 16-lane row sum, then
+// for two rows the other row's sum (IEEE addition commutes, so both rows agree)
+template <int PL>
+__device__ inline double group_sum(double v) {
+  v = row_sum(v);
+  if constexpr (PL >= 32) v += __shfl_xor(v, 16, 64);
+  if constexpr (PL == 64) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+// jacobi_round's lanes per pair: at most TN_SVD_LANES (16, 32 or 64), widened while a lane
+// keeps at least TN_SVD_LANE_ELEMS elements of a vector
+#ifndef TN_SVD_LANES
+#define TN_SVD_LANES 64
+#endif
+#ifndef TN_SVD_LANE_ELEMS
+#define TN_SVD_LANE_ELEMS 12
+#endif
 // global loads a thread keeps in flight while staging a block pair into LDS
 constexpr int kStage = 8;
 
@@ -93,11 +115,12 @@ __global__ __launch_bounds__(256) void jacobi_norms(const int64_t* __restrict__ 
 constexpr int kChunk = 8;
 
 // x_a <- x_a + ((c - 1) x_a - s x_b), x_b <- x_b + ((c - 1) x_b + s x_a) over n elements,
-// lane gl of the pair's P-lane group taking i = gl + P j.  Each chunk's loads are issued
+// lane gl of the pair's PL-lane group taking i = gl + PL j.  Each chunk's loads are issued
 // together; u0/v0 (when given) already hold the first chunk.
+template <int PL>
 __device__ inline void rotate_rows(double* xa, double* xb, int n, int gl, double cm1, double s,
                                    const double* u0, const double* v0) {
-  for (int base = 0; base < n; base += P * kChunk) {
+  for (int base = 0; base < n; base += PL * kChunk) {
     double u[kChunk], v[kChunk];
     if (base == 0 && u0 != nullptr) {
 #pragma unroll
@@ -108,7 +131,7 @@ __device__ inline void rotate_rows(double* xa, double* xb, int n, int gl, double
     } else {
 #pragma unroll
       for (int j = 0; j < kChunk; ++j) {
-        const int i = base + gl + P * j;
+        const int i = base + gl + PL * j;
         const int ii = i < n ? i : 0;
         u[j] = xa[ii];
         v[j] = xb[ii];
@@ -116,7 +139,7 @@ __device__ inline void rotate_rows(double* xa, double* xb, int n, int gl, double
     }
 #pragma unroll
     for (int j = 0; j < kChunk; ++j) {
-      const int i = base + gl + P * j;
+      const int i = base + gl + PL * j;
       if (i < n) {
         xa[i] = u[j] + fma(cm1, u[j], -s * v[j]);
         xb[i] = v[j] + fma(cm1, v[j], s * u[j]);
@@ -135,6 +158,7 @@ __device__ inline void pair_of(int n, int r, int p, int& a, int& b) {
   b = ib + 1;
 }
 
+template <int PL>
 __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* __restrict__ wgs,
                              double* __restrict__ x, double* __restrict__ q,
                              int32_t* __restrict__ rot_out, int w, int round, int first,
@@ -216,40 +240,54 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
       }
     }
   }
-  // one pair per 16-lane group: a round's pairs run side by side
-  const int gl = threadIdx.x % P;
-  const int grp = threadIdx.x / P;
-  const int ngrp = blockDim.x / P;
+  // one pair per PL-lane group: a round's pairs run side by side
+  const int gl = threadIdx.x % PL;
+  const int grp = threadIdx.x / PL;
+  const int ngrp = blockDim.x / PL;
   const double tol2 = tol * tol;
   const double floor2 = null_floor2(fro2, pw, null_scale);
+  // Round 0 of a sweep meets every block once, so its inner tournament (all pairs of the
+  // 2w vectors) visits the pairs inside every block of the problem.  Later rounds visit
+  // only the w x w pairs across their two blocks, w inner rounds of w disjoint pairs
+  // (slot p of the first block with slot w + (p + r) mod w of the second): every pair of
+  // the problem is visited exactly once per sweep, and a sweep costs about half the inner
+  // rounds of one that re-runs the pairs inside both blocks every round.
+  const bool cross = nb > 2 && round > 0;
+  const int nrounds = cross ? w : ne - 1;
+  const int npairs = cross ? w : ne / 2;
   int last = 0;
   for (int it = 0; it < inner_sweeps; ++it) {
     if (threadIdx.x == 0) rot_s = 0;
     __syncthreads();
     int mine = 0;
-    for (int r = 0; r < ne - 1; ++r) {
-      for (int p = grp; p < ne / 2; p += ngrp) {
+    for (int r = 0; r < nrounds; ++r) {
+      for (int p = grp; p < npairs; p += ngrp) {
         int a, b;
-        pair_of(ne, r, p, a, b);
+        if (cross) {
+          a = p;
+          b = w + (p + r >= w ? p + r - w : p + r);
+        } else {
+          pair_of(ne, r, p, a, b);
+        }
         double* xa = xs + a * L;
         double* xb = xs + b * L;
-        // a lane's elements i = gl + P j, taken kChunk at a time with every LDS load of a
+        // a lane's elements i = gl + PL j, taken kChunk at a time with every LDS load of a
         // chunk issued before its arithmetic (a lane past the end reads element 0 and
         // contributes zero); the first chunk stays in registers for the rotation
         double al = 0.0, be = 0.0, ga = 0.0;
         double u0[kChunk], v0[kChunk];
-        for (int base = 0; base < L; base += P * kChunk) {
+        for (int base = 0; base < L; base += PL * kChunk) {
           double u[kChunk], v[kChunk];
 #pragma unroll
           for (int j = 0; j < kChunk; ++j) {
-            const int i = base + gl + P * j;
+            const int i = base + gl + PL * j;
             const int ii = i < L ? i : 0;
             u[j] = xa[ii];
             v[j] = xb[ii];
           }
 #pragma unroll
           for (int j = 0; j < kChunk; ++j) {
-            const bool ok = base + gl + P * j < L;
+            const bool ok = base + gl + PL * j < L;
             const double uu = ok ? u[j] : 0.0, vv = ok ? v[j] : 0.0;
             al = fma(uu, uu, al);
             be = fma(vv, vv, be);
@@ -260,9 +298,9 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
             }
           }
         }
-        al = row_sum(al);
-        be = row_sum(be);
-        ga = row_sum(ga);
+        al = group_sum<PL>(al);
+        be = group_sum<PL>(be);
+        ga = group_sum<PL>(ga);
         // uniform within the group: every lane of it holds the same sums
         if (al > 0.0 && be > 0.0 && ga * ga > tol2 * al * be) {
           // t = tan(theta), |theta| <= pi/4, zeroes <x_a, x_b>; applied as
@@ -279,8 +317,8 @@ __global__ void jacobi_round(const int64_t* __restrict__ probs, const int32_t* _
           const double qd = 1.0 / (rr * den);
           const double s = t * den * qd;
           const double cm1 = -tt * rr * qd;
-          rotate_rows(xa, xb, L, gl, cm1, s, u0, v0);
-          rotate_rows(qs + a * k, qs + b * k, k, gl, cm1, s, nullptr, nullptr);
+          rotate_rows<PL>(xa, xb, L, gl, cm1, s, u0, v0);
+          rotate_rows<PL>(qs + a * k, qs + b * k, k, gl, cm1, s, nullptr, nullptr);
           mine += al > floor2 && be > floor2;
         }
       }
@@ -468,12 +506,27 @@ extern "C" int tn_svd_jacobi_round(const int64_t* probs, const int32_t* wgs, int
   const size_t bytes = static_cast<size_t>(2 * w) * static_cast<size_t>(span) * sizeof(double);
   if (span <= 0) return tn::fail("tn_svd_jacobi_round: empty launch span");
   if (fro2 == nullptr) return tn::fail("tn_svd_jacobi_round: no problem norms");
-  if (int rc = allow_lds(reinterpret_cast<const void*>(jacobi_round), bytes, "tn_svd_jacobi_round"))
-    return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int threads = w <= 4 ? 64 : (w >= 64 ? 1024 : P * w);  // one 16-lane group per pair
-  hipLaunchKernelGGL(jacobi_round, dim3(static_cast<unsigned>(nwgs)), dim3(threads), bytes, s, probs,
-                     wgs, x, q, rotations, w, round, first, tol, stop, fro2, null_scale);
+  // lanes per pair: more DPP rows per pair when the launch's vectors are long (span LDS
+  // doubles per vector), as many as keep at least TN_SVD_LANE_ELEMS elements per lane and
+  // one workgroup of at most 1024 threads
+  int pl = P;
+  while (pl < TN_SVD_LANES && 2 * pl * w <= 1024 && span >= 2 * pl * TN_SVD_LANE_ELEMS) pl *= 2;
+  const int threads = std::max(64, std::min(1024, pl * w));  // one pl-lane group per pair
+  const void* fn = pl == 64   ? reinterpret_cast<const void*>(jacobi_round<64>)
+                   : pl == 32 ? reinterpret_cast<const void*>(jacobi_round<32>)
+                              : reinterpret_cast<const void*>(jacobi_round<P>);
+  if (int rc = allow_lds(fn, bytes, "tn_svd_jacobi_round")) return rc;
+  const dim3 grid(static_cast<unsigned>(nwgs));
+  if (pl == 64)
+    hipLaunchKernelGGL(jacobi_round<64>, grid, dim3(threads), bytes, s, probs, wgs, x, q, rotations, w, round,
+                       first, tol, stop, fro2, null_scale);
+  else if (pl == 32)
+    hipLaunchKernelGGL(jacobi_round<32>, grid, dim3(threads), bytes, s, probs, wgs, x, q, rotations, w, round,
+                       first, tol, stop, fro2, null_scale);
+  else
+    hipLaunchKernelGGL(jacobi_round<P>, grid, dim3(threads), bytes, s, probs, wgs, x, q, rotations, w, round,
+                       first, tol, stop, fro2, null_scale);
   return tn::check_launch("jacobi_round");
 }
 
