@@ -170,6 +170,19 @@ int tn_axpby(int dtype, int64_t n, double alpha, const double* alpha_dev, int in
              const void* x, double beta, const double* beta_dev, int inv_beta, void* y,
              void* stream);
 
+/* Gram-Schmidt of a Krylov vector against the basis (the re-orthogonalising three-term
+ * step of network/dmrg.py:106-227 (lanczos), which there runs as one inner and one
+ * subtract per basis vector): `passes` (1 or 2) classical Gram-Schmidt passes of w against
+ * the k (1..8) basis vectors basis[0..k) (a host array of device pointers, dtype like w):
+ * c_i = <v_i, w>_W (weights W may be NULL), w <- w - sum_i c_i v_i.  coef[0..k) (device)
+ * receives the first pass's c_i and *last_coef (device, may be NULL) its c_{k-1}; *norm
+ * (device) receives |w|_W after the last pass.  FP64 accumulation, deterministic order;
+ * `work` must hold tn_krylov_orth_work() doubles.  Launches only (graph-capturable). */
+int tn_krylov_orth(int dtype, int64_t n, int k, const void* const* basis, void* w,
+                   const double* weights, int passes, double* coef, double* last_coef,
+                   double* norm, double* work, void* stream);
+int64_t tn_krylov_orth_work(void);
+
 /* x *= s (s host scalar times optional device scalar, optionally inverted). */
 int tn_scale(int dtype, int64_t n, double s, const double* s_dev, int inv, void* x,
              void* stream);
@@ -216,6 +229,7 @@ int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t* jobs, int6
  * tn_svd_jacobi_finish writes, per problem, S (k, descending), U (rows x k, row-major) and
  * Vh (k x cols, row-major) of the thin SVD. */
 #define TN_SVD_PROB_WORDS 11
+#define TN_LQ_PROB_WORDS 6
 #define TN_SVD_MAX_W 64
 #define TN_SVD_MAX_K 4096
 int tn_svd_jacobi_norms(const int64_t* probs, int64_t nprobs, const double* x, double* fro2,
@@ -232,6 +246,16 @@ int tn_svd_jacobi_converged(const int32_t* rotations, int64_t n, int32_t* stop, 
 int tn_svd_jacobi_finish(const int64_t* probs, int64_t nprobs, const double* x, const double* q,
                          double* U, double* S, double* Vh, const double* fro2, double null_scale,
                          int complete, void* stream);
+
+/* LQ of every sector matrix, replacing linalg.py:549-620 (lq: QR of each sector matrix's
+ * transpose per coupled sector).  Problem p (TN_LQ_PROB_WORDS int64: m, n, a_off, l_off,
+ * q_off, 0): A_p (m x n, row-major at A + a_off, overwritten by the Householder vectors)
+ * = L_p Q_p with k = min(m, n), L_p (m x k, lower trapezoidal, at L + l_off) and Q_p (k x n,
+ * orthonormal rows, at Q + q_off), FP64; the diagonal of L carries LAPACK dgeqrf's signs
+ * (beta_j = -sign(alpha) |x|).  One workgroup per problem; max_lds_doubles >= n + 2k of
+ * every problem. */
+int tn_lq(const int64_t* probs, int64_t nprobs, int64_t max_lds_doubles, double* A, double* L,
+          double* Q, void* stream);
 
 /* Ground eigenpair of the Lanczos tridiagonal matrix, on the device: replaces the
  * np.linalg.eigh(tri) of network/dmrg.py:216-222 (lanczos) and the host read of the Krylov

@@ -207,6 +207,132 @@ __global__ __launch_bounds__(kThreads) void mpo_kernel(const T* __restrict__ in,
                      base, R * S, static_cast<uint32_t>(S));
 }
 
+// ---- Gram-Schmidt pass of a Krylov vector against the basis ---------------------------
+// One classical Gram-Schmidt pass over k <= kOrthMax basis vectors: c_i = <v_i, w>_W for all
+// i in one sweep over the elements, then w <- w - sum_i c_i v_i in a second, which also
+// accumulates |w|_W^2 of the result.  Partials per workgroup, reduced by one workgroup in a
+// fixed order (deterministic, like tn_dot).
+constexpr int kOrthMax = 8;
+
+struct OrthBasis {
+  const void* v[kOrthMax];
+};
+
+// deterministic tree sum of one value per thread over the workgroup
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(kThreads) void multidot_partial(OrthBasis b, const T* __restrict__ w,
+                                                             const double* __restrict__ wt, int64_t n,
+                                                             double* __restrict__ part) {
+  __shared__ double red[kThreads];
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
+    double x = static_cast<double>(w[i]);
+    if (wt) x *= wt[i];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += static_cast<double>(static_cast<const T*>(b.v[k])[i]) * x;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double t = block_sum(acc[k], red);
+    if (threadIdx.x == 0) part[static_cast<int64_t>(blockIdx.x) * K + k] = t;
+  }
+}
+
+// coef[k] = sum over workgroups of part[g * K + k]; last (if given) = coef[K - 1]
+template <int K>
+__global__ __launch_bounds__(kThreads) void multidot_final(const double* __restrict__ part, int np,
+                                                           double* __restrict__ coef, double* __restrict__ last) {
+  __shared__ double red[kThreads];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double acc = 0.0;
+    for (int g = threadIdx.x; g < np; g += kThreads) acc += part[static_cast<int64_t>(g) * K + k];
+    const double t = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+      coef[k] = t;
+      if (last != nullptr && k == K - 1) *last = t;
+    }
+  }
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(kThreads) void cgs_update(OrthBasis b, T* __restrict__ w, const double* __restrict__ coef,
+                                                       const double* __restrict__ wt, int64_t n,
+                                                       double* __restrict__ part) {
+  __shared__ double red[kThreads];
+  double c[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) c[k] = coef[k];
+  double acc = 0.0;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n; i += stride) {
+    double x = static_cast<double>(w[i]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) x -= c[k] * static_cast<double>(static_cast<const T*>(b.v[k])[i]);
+    const T xs = static_cast<T>(x);
+    w[i] = xs;
+    double q = static_cast<double>(xs) * static_cast<double>(xs);
+    if (wt) q *= wt[i];
+    acc += q;
+  }
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void norm_final(const double* __restrict__ part, int np,
+                                                       double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  for (int g = threadIdx.x; g < np; g += kThreads) acc += part[g];
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) out[0] = sqrt(t);
+}
+
+template <typename T, int K>
+int orth_passes(OrthBasis b, T* w, const double* wt, int64_t n, int passes, double* coef, double* last,
+                double* norm, double* work, hipStream_t s) {
+  int64_t g = (n + kThreads * 8 - 1) / (kThreads * 8);
+  if (g < 1) g = 1;
+  if (g > kMaxPartials) g = kMaxPartials;
+  const unsigned gu = static_cast<unsigned>(g);
+  double* part = work;                                   // kOrthMax * kMaxPartials
+  double* coef2 = work + kOrthMax * kMaxPartials;        // kOrthMax
+  for (int p = 0; p < passes; ++p) {
+    double* c = p == 0 ? coef : coef2;
+    multidot_partial<T, K><<<gu, kThreads, 0, s>>>(b, w, wt, n, part);
+    multidot_final<K><<<1, kThreads, 0, s>>>(part, static_cast<int>(g), c, p == 0 ? last : nullptr);
+    cgs_update<T, K><<<gu, kThreads, 0, s>>>(b, w, c, wt, n, part);
+  }
+  norm_final<<<1, kThreads, 0, s>>>(part, static_cast<int>(g), norm);
+  return 0;
+}
+
+template <typename T, int K = 1>
+int orth_dispatch(int k, OrthBasis b, T* w, const double* wt, int64_t n, int passes, double* coef,
+                  double* last, double* norm, double* work, hipStream_t s) {
+  if constexpr (K > kOrthMax) {
+    return tn::fail("tn_krylov_orth: too many basis vectors");
+  } else {
+    if (k == K) return orth_passes<T, K>(b, w, wt, n, passes, coef, last, norm, work, s);
+    return orth_dispatch<T, K + 1>(k, b, w, wt, n, passes, coef, last, norm, work, s);
+  }
+}
+
 }  // namespace
 
 extern "C" int tn_dot(int dtype, const void* x, const void* y, const double* w, int64_t n,
@@ -293,4 +419,33 @@ extern "C" int tn_mpo_apply(int dtype, const void* in, void* out, const int64_t*
   else
     return tn::fail("tn_mpo_apply: unknown dtype");
   return tn::check_launch("tn_mpo_apply");
+}
+
+extern "C" int64_t tn_krylov_orth_work(void) {
+  return static_cast<int64_t>(kOrthMax) * kMaxPartials + kOrthMax;
+}
+
+extern "C" int tn_krylov_orth(int dtype, int64_t n, int k, const void* const* basis, void* w,
+                              const double* weights, int passes, double* coef, double* last_coef,
+                              double* norm, double* work, void* stream) {
+  if (k < 1 || k > kOrthMax) return tn::fail("tn_krylov_orth: k must lie in [1, 8]");
+  if (passes < 1 || passes > 2) return tn::fail("tn_krylov_orth: passes must be 1 or 2");
+  if (basis == nullptr || w == nullptr || coef == nullptr || norm == nullptr || work == nullptr)
+    return tn::fail("tn_krylov_orth: null pointer");
+  if (n <= 0) return tn::fail("tn_krylov_orth: empty vectors");
+  OrthBasis b{};
+  for (int i = 0; i < k; ++i) {
+    if (basis[i] == nullptr) return tn::fail("tn_krylov_orth: null basis vector");
+    b.v[i] = basis[i];
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc;
+  if (dtype == TN_F64)
+    rc = orth_dispatch<double>(k, b, static_cast<double*>(w), weights, n, passes, coef, last_coef, norm, work, s);
+  else if (dtype == TN_F32)
+    rc = orth_dispatch<float>(k, b, static_cast<float*>(w), weights, n, passes, coef, last_coef, norm, work, s);
+  else
+    return tn::fail("tn_krylov_orth: unknown dtype");
+  if (rc) return rc;
+  return tn::check_launch("tn_krylov_orth");
 }

@@ -460,6 +460,99 @@ __global__ void jacobi_finish(const int64_t* __restrict__ probs, const double* _
   }
 }
 
+// ---- LQ of sector matrices (Householder) ----------------------------------------------
+// One workgroup per matrix A (m x n, row-major, overwritten), k = min(m, n).  Step j forms
+// the reflector H_j = I - tau_j u_j u_j^T that maps row j's tail A[j][j:n) onto beta_j e_j
+// (u_j[j] = 1, the rest of u_j stored over A[j][j+1:n); beta_j = -sign(alpha) |x|, the
+// convention of LAPACK's dgeqrf on A^T), and applies it to the rows below.  Then
+// A = L Q with L[i][j] = A[i][j] (i > j), L[j][j] = beta_j, and Q = [I_k 0] H_{k-1} ... H_0,
+// formed by applying the reflectors to the unit rows in reverse order.  Rows are handled
+// by 16-lane groups (DPP row sums); the current reflector and the taus/betas sit in LDS.
+__global__ __launch_bounds__(256) void lq_kernel(const int64_t* __restrict__ probs, double* __restrict__ A,
+                                                 double* __restrict__ Lo, double* __restrict__ Qo) {
+  extern __shared__ double lds[];
+  __shared__ double part[4];
+  __shared__ double coef_s[2];
+  const int64_t* pw = probs + TN_LQ_PROB_WORDS * blockIdx.x;
+  const int m = static_cast<int>(pw[0]), n = static_cast<int>(pw[1]);
+  double* a = A + pw[2];
+  double* L = Lo + pw[3];
+  double* Q = Qo + pw[4];
+  const int k = m < n ? m : n;
+  double* v = lds;          // n: the reflector being applied
+  double* tau = lds + n;    // k
+  double* beta = tau + k;   // k
+  const int gl = threadIdx.x % P, grp = threadIdx.x / P, ngrp = blockDim.x / P;
+  for (int j = 0; j < k; ++j) {
+    double* rj = a + static_cast<int64_t>(j) * n;
+    double acc = 0.0;
+    for (int l = j + 1 + threadIdx.x; l < n; l += blockDim.x) acc = fma(rj[l], rj[l], acc);
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double xn2 = (part[0] + part[1]) + (part[2] + part[3]);
+      const double alpha = rj[j];
+      if (xn2 == 0.0) {  // nothing to annihilate: H_j = I
+        tau[j] = 0.0;
+        beta[j] = alpha;
+        coef_s[0] = 0.0;
+      } else {
+        const double b = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
+        tau[j] = (b - alpha) / b;
+        beta[j] = b;
+        coef_s[0] = 1.0 / (alpha - b);
+      }
+    }
+    __syncthreads();
+    const double scale = coef_s[0], tj = tau[j];
+    for (int l = j + 1 + threadIdx.x; l < n; l += blockDim.x) {
+      const double u = rj[l] * scale;
+      v[l] = u;
+      rj[l] = u;
+    }
+    if (threadIdx.x == 0) v[j] = 1.0;
+    __syncthreads();
+    if (tj != 0.0) {
+      for (int i = j + 1 + grp; i < m; i += ngrp) {
+        double* ri = a + static_cast<int64_t>(i) * n;
+        double s = 0.0;
+        for (int l = j + gl; l < n; l += P) s = fma(ri[l], v[l], s);
+        const double ts = tj * row_sum(s);
+        for (int l = j + gl; l < n; l += P) ri[l] = fma(-ts, v[l], ri[l]);
+      }
+    }
+    __syncthreads();
+  }
+  // L (m x k): the columns below the diagonal, beta on it, zeros above
+  for (int e = threadIdx.x; e < m * k; e += blockDim.x) {
+    const int i = e / k, c = e - (e / k) * k;
+    L[e] = c < i ? a[static_cast<int64_t>(i) * n + c] : (c == i ? beta[c] : 0.0);
+  }
+  // Q (k x n) = [I_k 0] H_{k-1} ... H_0: unit rows, then each reflector applied on the right
+  for (int e = threadIdx.x; e < k * n; e += blockDim.x) {
+    const int i = e / n, c = e - (e / n) * n;
+    Q[e] = c == i ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int j = k - 1; j >= 0; --j) {
+    const double tj = tau[j];
+    if (tj == 0.0) continue;  // uniform: tau sits in LDS
+    const double* rj = a + static_cast<int64_t>(j) * n;
+    for (int l = j + 1 + threadIdx.x; l < n; l += blockDim.x) v[l] = rj[l];
+    if (threadIdx.x == 0) v[j] = 1.0;
+    __syncthreads();
+    for (int i = grp; i < k; i += ngrp) {
+      double* qi = Q + static_cast<int64_t>(i) * n;
+      double s = 0.0;
+      for (int l = j + gl; l < n; l += P) s = fma(qi[l], v[l], s);
+      const double ts = tj * row_sum(s);
+      for (int l = j + gl; l < n; l += P) qi[l] = fma(-ts, v[l], qi[l]);
+    }
+    __syncthreads();
+  }
+}
+
 // Dynamic LDS above the default needs the per-function opt-in; the request must stay
 // within the device's per-workgroup limit (less what the kernel declares statically).
 int allow_lds(const void* fn, size_t bytes, const char* who) {
@@ -543,4 +636,16 @@ extern "C" int tn_svd_jacobi_finish(const int64_t* probs, int64_t nprobs, const 
   hipLaunchKernelGGL(jacobi_finish, dim3(static_cast<unsigned>(nprobs)), dim3(1024), bytes, s, probs,
                      x, q, U, S, Vh, fro2, null_scale, complete);
   return tn::check_launch("jacobi_finish");
+}
+
+extern "C" int tn_lq(const int64_t* probs, int64_t nprobs, int64_t max_lds_doubles, double* A, double* L,
+                     double* Q, void* stream) {
+  if (nprobs <= 0) return 0;
+  if (nprobs > 0x7fffffff) return tn::fail("tn_lq: too many problems");
+  if (max_lds_doubles <= 0) return tn::fail("tn_lq: empty LDS request");
+  const size_t bytes = static_cast<size_t>(max_lds_doubles) * sizeof(double);
+  if (int rc = allow_lds(reinterpret_cast<const void*>(lq_kernel), bytes, "tn_lq")) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(lq_kernel, dim3(static_cast<unsigned>(nprobs)), dim3(256), bytes, s, probs, A, L, Q);
+  return tn::check_launch("tn_lq");
 }

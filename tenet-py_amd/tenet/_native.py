@@ -23,6 +23,7 @@ PROB_WORDS = 8
 SEG_WORDS = 6
 MPO_JOB_WORDS = 8
 MPO_TILE = 1024  # positions per tn_mpo_apply tile (256 threads x 4)
+LQ_PROB_WORDS = 6
 SVD_PROB_WORDS = 11
 SVD_MAX_W = 64
 SVD_MAX_K = 4096
@@ -47,6 +48,9 @@ _SIGNATURES = {
     "tn_gemm_grouped_rows": (_i32, [_i32, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _p]),
     "tn_gemm_skinny": (_i32, [_i32, _i32, _i32, _i32, _p, _p, _p, _p, _p, _p, _p, _i64, _p]),
     "tn_dot": (_i32, [_i32, _p, _p, _p, _i64, _p, _p, _p]),
+    "tn_krylov_orth": (_i32, [_i32, _i64, _i32, _p, _p, _p, _i32, _p, _p, _p, _p, _p]),
+    "tn_krylov_orth_work": (_i64, []),
+    "tn_lq": (_i32, [_p, _i64, _i64, _p, _p, _p, _p]),
     "tn_axpby": (_i32, [_i32, _i64, _d, _p, _i32, _p, _d, _p, _i32, _p, _p]),
     "tn_scale": (_i32, [_i32, _i64, _d, _p, _i32, _p, _p]),
     "tn_cast": (_i32, [_i32, _p, _i32, _p, _i64, _p]),

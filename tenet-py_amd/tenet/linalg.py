@@ -601,18 +601,61 @@ def svd_truncated_with_selection(t, axes=None, *, max_bond=None, cutoff=None, cu
     return (*_factors(m, parts, keep, sel.scale), sel)
 
 
+class _LQPlan:
+    """Problem table of ``tn_lq`` for one packed matrix layout."""
+
+    def __init__(self, ml: _MatrixLayout) -> None:
+        from tenet import _native
+        from tenet.engine import _DeviceCache
+
+        rows, self.slices = [], {}
+        lo = qo = 0
+        lds = 1
+        for c in ml.sectors:
+            m, n = ml.shapes[c]
+            k = min(m, n)
+            self.slices[c] = (lo, m, k, qo, n)
+            if k:
+                rows.append([m, n, ml.offsets[c], lo, qo, 0])
+                lds = max(lds, n + 2 * k)
+            lo += m * k
+            qo += k * n
+        if lds * 8 > _lds_budget():
+            raise ValueError(f"lq: a sector matrix needs {lds} doubles of LDS (columns + 2 min(rows, cols)); "
+                             f"the kernel has {_lds_budget() // 8}")
+        self.l_size, self.q_size, self.lds, self.nprobs = lo, qo, lds, len(rows)
+        self.probs = _DeviceCache(probs=np.asarray(rows, dtype=np.int64).reshape(-1))
+        assert not rows or len(rows[0]) == _native.LQ_PROB_WORDS
+
+
 def lq(t: SymmetricTensor, axes=None):
     """``T = L Q`` per coupled sector with ``Q`` row-orthonormal, bond ``min(rows, cols)``
-    (reference ``linalg.py:549-620``: QR of each sector matrix's transpose)."""
-    m, mats, _ = _lower(t, axes)
+    (reference ``linalg.py:549-620``: QR of each sector matrix's transpose; L's diagonal
+    carries LAPACK's signs).  All sectors in one ``tn_lq`` launch (FP64 Householder; an
+    FP32 tensor is factorised in FP64 and the factors cast back)."""
+    from tenet import _native
+
+    m, mats, (buf, ml) = _lower(t, axes)
     provider = m.provider
     keep = {c: min(b.shape) for c, b in mats.items()}
     bond = GradedSpace.new(provider, keep)
+    plan = getattr(ml, "_lq", None)
+    if plan is None:
+        plan = ml._lq = _LQPlan(ml)
+    dev = buf.device
+    a = buf if buf.dtype == torch.float64 else buf.to(torch.float64)
+    Lb = torch.empty(max(plan.l_size, 1), dtype=torch.float64, device=dev)
+    Qb = torch.empty(max(plan.q_size, 1), dtype=torch.float64, device=dev)
+    if plan.nprobs:
+        _native.check(_native.lib().tn_lq(plan.probs.get(dev)["probs"].data_ptr(), plan.nprobs, plan.lds,
+                                          a.data_ptr(), Lb.data_ptr(), Qb.data_ptr(), _native.stream_ptr()))
+    if buf.dtype != torch.float64:
+        Lb, Qb = Lb.to(buf.dtype), Qb.to(buf.dtype)
     ls, qs = {}, {}
-    for c, b in mats.items():
-        q, r = torch.linalg.qr(b.transpose(0, 1))
-        ls[c] = r.transpose(0, 1)
-        qs[c] = q.transpose(0, 1)
+    for c in ml.sectors:
+        lo, rows, k, qo, cols = plan.slices[c]
+        ls[c] = Lb[lo: lo + rows * k].view(rows, k)
+        qs[c] = Qb[qo: qo + k * cols].view(k, cols)
     L = from_matrices(TensorStructure((*m.codomain, Leg(bond, IN))), ls)
     Q = from_matrices(TensorStructure((Leg(bond, OUT), *m.domain)), qs)
     return L, Q

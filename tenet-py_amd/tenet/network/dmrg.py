@@ -86,14 +86,22 @@ def lanczos(matvec: Callable[[SymmetricTensor], SymmetricTensor], v: SymmetricTe
     vecs = [ops.divide(v, ops.norm(v))]
     for j in range(ncv):
         w = matvec(vecs[j])
-        a = ops.inner_device(vecs[j], w, alphas[j])
-        w = ops.subtract(w, ops.multiply(vecs[j], a))
-        if j:
-            w = ops.subtract(w, ops.multiply(vecs[j - 1], betas[j - 1]))
-        if reorthogonalize:
-            for u in vecs:
-                w = ops.subtract(w, ops.multiply(u, ops.inner(u, w)))
-        betas[j] = ops.norm(w)
+        if reorthogonalize and j + 1 <= ops.ORTH_MAX and w.dtype == vecs[j].dtype:
+            # the three-term step and its re-orthogonalisation as two classical
+            # Gram-Schmidt passes against the whole basis (one native call): the first
+            # pass's coefficient of v_j is alpha_j, the norm after the second is beta_j
+            if any(w.data.data_ptr() == u.data.data_ptr() for u in vecs):
+                w = ops.multiply(w, 1.0)
+            ops.orthogonalize_(w, vecs, passes=2, last_coef=alphas[j], norm=betas[j])
+        else:
+            a = ops.inner_device(vecs[j], w, alphas[j])
+            w = ops.subtract(w, ops.multiply(vecs[j], a))
+            if j:
+                w = ops.subtract(w, ops.multiply(vecs[j - 1], betas[j - 1]))
+            if reorthogonalize:
+                for u in vecs:
+                    w = ops.subtract(w, ops.multiply(u, ops.inner(u, w)))
+            betas[j] = ops.norm(w)
         if j + 1 == ncv:
             break
         vecs.append(ops.divide(w, betas[j]))

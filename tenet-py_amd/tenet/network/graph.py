@@ -6,9 +6,9 @@ Iteration ``j`` of the recurrence of ``tenet.network.lanczos`` (reference
 operator and the vector's structure are fixed:
 
     w = H v_j                                   (the H_eff launches)
-    alpha_j = <v_j|w>;  w -= alpha_j v_j + beta_{j-1} v_{j-1}
-    w -= sum_k <v_k|w> v_k   (k <= j, FP64 re-orthogonalisation)
-    beta_j = |w|;  v_{j+1} = w / beta_j
+    c = V^T w;  w -= V c;  d = V^T w;  w -= V d   (V = v_0..v_j, two classical Gram-Schmidt
+                                                 passes in one tn_krylov_orth call)
+    alpha_j = c_j;  beta_j = |w|;  v_{j+1} = w / beta_j
 
 so it is captured once as a hipGraph and replayed.  Every Krylov scalar lives in device
 memory; the host reads ``alpha``/``beta`` once after the last replay, solves the small
@@ -22,6 +22,8 @@ their buffers: it belongs to one compiled ``Heff2`` and one vector structure.
 """
 
 from __future__ import annotations
+
+import ctypes
 
 import torch
 
@@ -84,7 +86,8 @@ class KrylovGraph:
         self.V = torch.zeros(ncv + 1, n, **f64)
         self.alphas = torch.zeros(ncv, **f64)
         self.betas = torch.zeros(ncv, **f64)
-        self.coef = torch.zeros(ncv + 1, **f64)
+        self.coef = torch.zeros(max(ncv + 1, ops.ORTH_MAX), **f64)
+        self.owork = torch.empty(int(_native.lib().tn_krylov_orth_work()), **f64)
         self.g = torch.zeros(ncv, **f64)
         self.value = torch.zeros((), **f64)
         self.out = torch.zeros(n, **f64)
@@ -118,16 +121,26 @@ class KrylovGraph:
         w = self.op.apply_data(self.structure, v)
         if w.dtype != torch.float64:
             raise TypeError("KrylovGraph expects an FP64 matvec result")
-        _dot(v, w, self.weights, self.work, self.alphas[j])
-        _axpby(-1.0, self.alphas[j], 0, v, 1.0, w)
-        if j:
-            _axpby(-1.0, self.betas[j - 1], 0, self.V[j - 1], 1.0, w)
-        if self.reorth:
-            for k in range(j + 1):
-                _dot(self.V[k], w, self.weights, self.work, self.coef[k])
-                _axpby(-1.0, self.coef[k], 0, self.V[k], 1.0, w)
-        _dot(w, w, self.weights, self.work, self.tmp)
-        torch.sqrt(self.tmp, out=self.betas[j])
+        if self.reorth and j + 1 <= ops.ORTH_MAX:
+            # two classical Gram-Schmidt passes against V[0..j] in one native call: the
+            # first pass's coefficient of V[j] is alpha_j, the norm after the second beta_j
+            ptrs = (ctypes.c_void_p * (j + 1))(*[self.V[k].data_ptr() for k in range(j + 1)])
+            _native.check(_native.lib().tn_krylov_orth(
+                _native.TN_F64, w.numel(), j + 1, ptrs, w.data_ptr(),
+                None if self.weights is None else self.weights.data_ptr(), 2, self.coef.data_ptr(),
+                self.alphas[j].data_ptr(), self.betas[j].data_ptr(), self.owork.data_ptr(),
+                _native.stream_ptr()))
+        else:
+            _dot(v, w, self.weights, self.work, self.alphas[j])
+            _axpby(-1.0, self.alphas[j], 0, v, 1.0, w)
+            if j:
+                _axpby(-1.0, self.betas[j - 1], 0, self.V[j - 1], 1.0, w)
+            if self.reorth:
+                for k in range(j + 1):
+                    _dot(self.V[k], w, self.weights, self.work, self.coef[k])
+                    _axpby(-1.0, self.coef[k], 0, self.V[k], 1.0, w)
+            _dot(w, w, self.weights, self.work, self.tmp)
+            torch.sqrt(self.tmp, out=self.betas[j])
         _axpby(1.0, self.betas[j], 1, w, 0.0, self.V[j + 1])
 
     def _finish(self) -> None:

@@ -8,6 +8,7 @@ kernels (``tenet.engine``).  Nothing here touches a block on the CPU.
 
 from __future__ import annotations
 
+import ctypes
 from collections import Counter
 from dataclasses import replace
 from functools import lru_cache
@@ -276,6 +277,48 @@ def _work(device) -> torch.Tensor:
     if w is None:
         w = _WORK[key] = torch.empty(1024, dtype=torch.float64, device=device)
     return w
+
+
+ORTH_MAX = 8  # basis vectors one tn_krylov_orth call takes
+_ORTH_WORK: dict = {}
+
+
+def _orth_work(device) -> torch.Tensor:
+    """Partials buffer of ``tn_krylov_orth``, one per (device, stream)."""
+    key = (device.type, device.index, _native.stream_ptr())
+    w = _ORTH_WORK.get(key)
+    if w is None:
+        n = int(_native.lib().tn_krylov_orth_work())
+        w = _ORTH_WORK[key] = torch.empty(n, dtype=torch.float64, device=device)
+    return w
+
+
+def orthogonalize_(w: SymmetricTensor, basis, *, passes: int = 2, coef: torch.Tensor | None = None,
+                   last_coef: torch.Tensor | None = None, norm: torch.Tensor | None = None):
+    """Gram-Schmidt of ``w`` against the (orthonormal) ``basis``, in place: ``passes``
+    classical passes ``w <- w - sum_i <v_i, w> v_i`` over every basis vector at once
+    (qdim-weighted inner products), with FP64 accumulation (``tn_krylov_orth``).  Returns
+    ``(coef, norm)``: the first pass's coefficients and ``|w|`` after the update, as device
+    tensors; ``last_coef`` (a 0-d device tensor) also receives the last basis vector's
+    coefficient.  At most ``ORTH_MAX`` basis vectors."""
+    basis = list(basis)
+    if not 1 <= len(basis) <= ORTH_MAX:
+        raise ValueError(f"orthogonalize_: 1..{ORTH_MAX} basis vectors, got {len(basis)}")
+    for v in basis:
+        _check_same(v, w, "orthogonalize_")
+    require_device(w, *basis)
+    dev = w.device
+    if any(v.data.data_ptr() == w.data.data_ptr() for v in basis):
+        raise ValueError("orthogonalize_: w shares its storage with a basis vector")
+    coef = torch.empty(ORTH_MAX, dtype=torch.float64, device=dev) if coef is None else coef
+    norm = torch.empty((), dtype=torch.float64, device=dev) if norm is None else norm
+    ptrs = (ctypes.c_void_p * len(basis))(*[v.data.data_ptr() for v in basis])
+    wt = _weights(w.structure, dev)
+    _native.check(_native.lib().tn_krylov_orth(
+        _dt(w), w.data.numel(), len(basis), ptrs, w.data.data_ptr(), None if wt is None else wt.data_ptr(),
+        passes, coef.data_ptr(), None if last_coef is None else last_coef.data_ptr(), norm.data_ptr(),
+        _orth_work(dev).data_ptr(), _native.stream_ptr()))
+    return coef, norm
 
 
 def inner_device(a: SymmetricTensor, b: SymmetricTensor, out: torch.Tensor | None = None):
