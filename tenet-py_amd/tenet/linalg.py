@@ -251,6 +251,11 @@ _SVD_INNER_SWEEPS = 1     # inner sweeps per block-pair visit when a problem spa
 _SVD_SINGLE_SWEEPS = 64   # cap for a problem that fits in one block pair (converges in-launch)
 _SVD_LAST_SWEEPS = 0      # sweeps queued by the last call (diagnostics)
 _SVD_SWEEP_BATCH = 8      # sweeps queued before the first read of the convergence flags
+# widest block of a block tournament (vectors): a round's time follows the pairs one
+# workgroup rotates, and the sweep's depth (about k inner rounds) does not depend on the
+# width, so narrower blocks spread the same rounds over more workgroups (16 measured
+# faster than 32 on 105-260-vector sectors; 8 and 4 no faster or slower)
+_SVD_MAX_BLOCK = 16
 
 
 def _lds_budget() -> int:
@@ -272,11 +277,12 @@ def _single_pair(k: int, L: int) -> bool:
 
 
 def _svd_width(span: int, k: int) -> int:
-    """Vectors per block of the block tournament: the largest power of two <= 32 whose
-    block pair (2w vectors of length L and their Q rows of length k, span = L + k doubles
-    each) fits in LDS, and no wider than the smallest pair that holds all k vectors."""
+    """Vectors per block of the block tournament: the largest power of two up to
+    ``_SVD_MAX_BLOCK`` whose block pair (2w vectors of length L and their Q rows of length
+    k, span = L + k doubles each) fits in LDS, and no wider than the smallest pair that
+    holds all k vectors."""
     budget = _lds_budget()
-    w = 32
+    w = _SVD_MAX_BLOCK
     while w > 1 and (2 * w * span * 8 > budget or w >= k):
         w //= 2
     if 2 * w * span * 8 > budget:
