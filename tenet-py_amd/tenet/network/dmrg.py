@@ -430,7 +430,7 @@ def dmrg_(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1
           refine: bool = True, polish_tol: float | None = None) -> DMRG_out:
     """Sweep ``psi`` to the ground state of ``h`` in place (reference ``dmrg.py:620-843``)."""
     if orthogonal_to:
-        raise NotImplementedError("excited-state projection is not part of this build")
+        raise NotImplementedError("excited-state projection (orthogonal_to) is not implemented")
     gen = dmrg_steps(psi, h, schedule=schedule, chi=chi, cutoff=cutoff, energy_tol=energy_tol,
                      schmidt_tol=schmidt_tol, max_sweeps=max_sweeps, ncv=ncv, seed=seed,
                      callback=callback, compile=compile, precision=precision, refine=refine,

@@ -81,14 +81,14 @@ def _decode_leg(d: dict) -> Leg:
         raise KeyError(f"unknown provider kind {kind!r}; known kinds are {sorted(_PROVIDERS)}")
     if prov.get("name", kind) != kind:
         raise ValueError(f"provider {kind!r} renamed to {prov['name']!r} is a different symmetry; "
-                         "this build only has the default-named providers")
+                         "only the default-named providers are available here")
     space = GradedSpace.new(_PROVIDERS[kind],
                             {_SECTOR_OF[kind](args): int(m) for args, m in d["space"]["sectors"]})
     return Leg(space, OUT if d["side"] == "out" else IN, bool(d["dual"]), d.get("name"))
 
 
 def _gauge_accepted(kind: str, gauge: str) -> bool:
-    """This build's own fingerprint, or -- for SU(2) -- the reference's racah-derived one:
+    """The running library's own fingerprint, or -- for SU(2) -- the reference's racah-derived one:
     the coefficient tables here agree with racah's on every row of the reference's F and R
     fixtures (to 4.4e-16), so such files are numerically comparable."""
     running = GAUGES.get(kind)
@@ -119,12 +119,12 @@ def load(path: str | os.PathLike, *, device=None, dtype=torch.float64):
     with np.load(path, allow_pickle=False) as z:
         header = json.loads(str(z["header"].item()))
         if header["format"] > FORMAT_VERSION:
-            raise ValueError(f"{path}: file format {header['format']} is newer than this build's "
+            raise ValueError(f"{path}: file format {header['format']} is newer than this library's "
                              f"format {FORMAT_VERSION}")
         for kind, gauge in header.get("gauges", {}).items():
             running = GAUGES.get(kind)
             if not _gauge_accepted(kind, gauge):
-                raise ValueError(f"{path}: {kind} coefficient gauge on disk is {gauge!r}, but this build "
+                raise ValueError(f"{path}: {kind} coefficient gauge on disk is {gauge!r}, but this library "
                                  f"runs {running!r}; the block coefficients are not comparable")
         structure = TensorStructure(tuple(_decode_leg(d) for d in header["legs"]))
         n = header["num_blocks"]

@@ -9,7 +9,7 @@
   agreement of the two trees (reference ``structure.py:349-371``).
 * ``block_shape`` -- degeneracies in *public axis order*.
 
-On top of that this build fixes the device layout: every block is C-contiguous and the
+On top of that the device layout is fixed: every block is C-contiguous and the
 blocks follow each other in ``block_order`` in ONE flat buffer, so ``offsets[i]`` (an
 int64 table, computed once per structure) locates block ``i``.
 

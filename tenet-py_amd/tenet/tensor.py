@@ -222,7 +222,7 @@ class SymmetricTensor:
 
     def to_backend(self, backend: str = "torch") -> "SymmetricTensor":
         if backend != "torch":
-            raise ValueError("this build keeps blocks in torch-ROCm device memory only")
+            raise ValueError("blocks live in torch-ROCm device memory only")
         return self
 
     # ---------------------------------------------------------------- algebra
