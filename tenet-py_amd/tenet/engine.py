@@ -35,7 +35,7 @@ from tenet.plans import (
     permutation_plan,
     repartition_plan,
 )
-from tenet.structure import FusionBlockKey, TensorStructure
+from tenet.structure import FusionBlockKey, TensorStructure, pattern_of
 
 __all__ = [
     "RecoupleProgram",
@@ -862,38 +862,27 @@ class ContractionExec:
         else:
             self.c_program = None
             c_size = cur.size
-        # ---- block problems of the composition C' = A' o B'
-        shapes_a, shapes_b = Ap.shapes, Bp.shapes
+        # ---- block problems of the composition C' = A' o B': which A and B blocks meet in
+        # which C block is fixed by the sector pattern (shared by every structure of it);
+        # only the extents and offsets are this structure's own
+        shapes_a = Ap.shapes
         shapes_c = Cp.shapes
-        b_index = {}
-        for j, key in enumerate(Bp.block_order):
-            b_index.setdefault(key.coupled, {}).setdefault(key.output_tree, []).append((key.input_tree, j))
-        a_rows = {}
-        for j, key in enumerate(Ap.block_order):
-            a_rows.setdefault((key.coupled, key.output_tree), []).append((key.input_tree, j))
-        b_lookup = {}
-        for j, key in enumerate(Bp.block_order):
-            b_lookup[(key.output_tree, key.input_tree)] = j
+        ptr, seg_a, seg_b = _block_pairing(pattern_of(Ap), pattern_of(Bp), pattern_of(Cp))
         # row / column / inner extents of every block, one vectorised product each
         sc = np.asarray(shapes_c, dtype=np.int64).reshape(len(shapes_c), Cp.ndim)
         sa_ = np.asarray(shapes_a, dtype=np.int64).reshape(len(shapes_a), Ap.ndim)
-        Ms = np.prod(sc[:, :n_out_a], axis=1).tolist()
-        Ns = np.prod(sc[:, n_out_a:], axis=1).tolist()
-        Ka = np.prod(sa_[:, n_out_a:], axis=1).tolist()
-        a_offs = np.asarray(self.a.offsets).tolist()
-        b_offs = np.asarray(self.b.offsets).tolist()
-        c_list = np.asarray(c_offs).tolist()
-        problems = []
-        for i, key in enumerate(Cp.block_order):
-            M, N = Ms[i], Ns[i]
-            segs = []
-            for kt, ja in a_rows.get((key.coupled, key.output_tree), ()):
-                jb = b_lookup.get((kt, key.input_tree))
-                if jb is None:
-                    continue
-                K = Ka[ja]
-                segs.append((a_offs[ja], b_offs[jb], K, K, N))
-            problems.append((M, N, c_list[i], N, segs))
+        Ms = np.prod(sc[:, :n_out_a], axis=1)
+        Ns = np.prod(sc[:, n_out_a:], axis=1)
+        Ka = np.prod(sa_[:, n_out_a:], axis=1)
+        seg_c = np.repeat(np.arange(len(Ms)), np.diff(ptr))
+        seg_rows = zip(np.asarray(self.a.offsets, dtype=np.int64)[seg_a].tolist(),
+                       np.asarray(self.b.offsets, dtype=np.int64)[seg_b].tolist(),
+                       Ka[seg_a].tolist(), Ns[seg_c].tolist())
+        segs_all = [(ao, bo, K, K, N) for ao, bo, K, N in seg_rows]
+        ptr_l = ptr.tolist()
+        problems = [(M, N, c, N, segs_all[lo:hi])
+                    for M, N, c, lo, hi in zip(Ms.tolist(), Ns.tolist(), np.asarray(c_offs).tolist(),
+                                               ptr_l[:-1], ptr_l[1:])]
         a_size = sa.size if self.a.program is None else Ap.size
         b_size = sb.size if self.b.program is None else Bp.size
         self.gemm = GemmProgram(problems, a_size, b_size, c_size)
@@ -934,6 +923,27 @@ class ContractionExec:
         if self.c_program is not None:
             self.c_program.run(target, out)
         return out
+
+
+@lru_cache(maxsize=1024)
+def _block_pairing(pa: TensorStructure, pb: TensorStructure, pc: TensorStructure) -> tuple:
+    """Segments of the block problems of ``C = A o B`` on sector patterns: for C block i,
+    the A/B block pairs ``(seg_a[s], seg_b[s])`` for s in ``[ptr[i], ptr[i+1])``, A's input
+    tree matching B's output tree, in A's block order."""
+    a_rows = {}
+    for j, key in enumerate(pa.block_order):
+        a_rows.setdefault((key.coupled, key.output_tree), []).append((key.input_tree, j))
+    b_lookup = {(key.output_tree, key.input_tree): j for j, key in enumerate(pb.block_order)}
+    ptr, seg_a, seg_b = [0], [], []
+    for key in pc.block_order:
+        for kt, ja in a_rows.get((key.coupled, key.output_tree), ()):
+            jb = b_lookup.get((kt, key.input_tree))
+            if jb is not None:
+                seg_a.append(ja)
+                seg_b.append(jb)
+        ptr.append(len(seg_a))
+    return (np.asarray(ptr, dtype=np.int64), np.asarray(seg_a, dtype=np.int64),
+            np.asarray(seg_b, dtype=np.int64))
 
 
 @lru_cache(maxsize=2048)

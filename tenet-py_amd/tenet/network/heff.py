@@ -23,7 +23,7 @@ from tenet._native import MPO_TILE
 
 from tenet import ops
 from tenet.engine import contraction_exec, inplace_map, recouple_program, require_device
-from tenet.plans import repartition_plan
+from tenet.plans import identity_map, repartition_plan
 from tenet.space import OUT
 from tenet.structure import TensorStructure, _axis_index
 from tenet.tensor import SymmetricTensor
@@ -54,7 +54,7 @@ def _einsum_axes(t0: str, t1: str, out: str):
 
 
 def _identity_plan(S: TensorStructure):
-    return tuple(range(S.ndim)), tuple((i, i, 1.0) for i in range(S.num_blocks))
+    return tuple(range(S.ndim)), identity_map(S.num_blocks)
 
 
 def _boundary(S: TensorStructure, D: TensorStructure, plan):

@@ -52,6 +52,7 @@ __all__ = [
 # ----------------------------------------------------------------------------- block maps
 
 
+@cache
 def identity_map(n: int) -> tuple:
     return tuple((i, i, 1.0) for i in range(n))
 
@@ -65,7 +66,30 @@ def _unit_bijection(terms) -> bool:
     return len(srcs) == len(dsts) == len(terms)
 
 
+_COMPOSED: dict = {}  # (id(first), id(second)) -> (first, second, composition)
+_COMPOSED_MAX = 8192
+
+
 def compose_maps(first, second) -> tuple:
+    """``_compose_maps`` remembered by the identity of its operands: the block maps come
+    from pattern-level plans (one object per sector pattern, shared by every structure of
+    that pattern), so the structures of consecutive bond steps, whose degeneracies change
+    but whose sector patterns do not, compose the same two objects again.  The entry keeps
+    both operands alive, so an id is never reused while its entry exists."""
+    if type(first) is not tuple or type(second) is not tuple:
+        return _compose_maps(first, second)
+    key = (id(first), id(second))
+    hit = _COMPOSED.get(key)
+    if hit is not None and hit[0] is first and hit[1] is second:
+        return hit[2]
+    out = _compose_maps(first, second)
+    if len(_COMPOSED) >= _COMPOSED_MAX:
+        _COMPOSED.clear()
+    _COMPOSED[key] = (first, second, out)
+    return out
+
+
+def _compose_maps(first, second) -> tuple:
     """The block map of applying ``first`` and then ``second``: every path
     ``src -> mid -> dst`` contributes ``c_first * c_second``, contributions to one
     ``(src, dst)`` pair summed in the order the paths are met (first's terms in order,
