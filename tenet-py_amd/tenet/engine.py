@@ -423,24 +423,32 @@ def _xcd_layout(tiles: np.ndarray, nprob: int, nxcd: int = 8, grid: int = 0) -> 
     work = tiles[:, 3].astype(np.float64) + 64.0  # K plus a per-tile fixed cost
     cap = max(work.sum() / (nxcd * 6), 1.0)
     order = np.lexsort((tiles[:, 2], tiles[:, 1], tiles[:, 0]))
-    units, cur, cw, key = [], [], 0.0, None
-    for idx in order:
-        p, ti = int(tiles[idx, 0]), int(tiles[idx, 1])
-        if cur and (p != key[0] or cw >= cap):
-            units.append(cur)
-            cur, cw = [], 0.0
-        cur.append(idx)
-        cw += work[idx]
-        key = (p, ti)
-    if cur:
-        units.append(cur)
-    units.sort(key=lambda u: -work[u].sum())
+    # a unit closes once its work reaches cap or its problem ends; every tile of a problem
+    # carries the same K, so a unit holds n tiles (n w >= cap, n minimal; integer-valued
+    # work, so the products are exact) -- the units follow from the problem boundaries
+    # without a walk over the tiles
+    ps = tiles[order, 0]
+    starts = np.flatnonzero(np.r_[True, ps[1:] != ps[:-1]]) if len(order) else np.zeros(0, np.int64)
+    ends = np.r_[starts[1:], len(order)]
+    spans, uwork = [], []
+    for s0, e0 in zip(starts.tolist(), ends.tolist()):
+        w = float(work[order[s0]])
+        n = max(1, int(np.ceil(cap / w)))
+        while n > 1 and (n - 1) * w >= cap:
+            n -= 1
+        while n * w < cap:
+            n += 1
+        for a in range(s0, e0, n):
+            b = min(e0, a + n)
+            spans.append((a, b))
+            uwork.append((b - a) * w)
     loads = np.zeros(nxcd)
     parts = [[] for _ in range(nxcd)]
-    for u in units:
+    for u in sorted(range(len(spans)), key=lambda i: -uwork[i]):
         x = int(np.argmin(loads))
-        parts[x].extend(u)
-        loads[x] += work[u].sum()
+        a, b = spans[u]
+        parts[x].extend(order[a:b].tolist())
+        loads[x] += uwork[u]
     if grid > 0:
         per = grid // nxcd
         rounds = max(-(-len(pt) // per) for pt in parts)
