@@ -469,7 +469,8 @@ __global__ void jacobi_finish(const int64_t* __restrict__ probs, const double* _
 // formed by applying the reflectors to the unit rows in reverse order.  Rows are handled
 // by 16-lane groups (DPP row sums); the current reflector and the taus/betas sit in LDS.
 __global__ __launch_bounds__(256) void lq_kernel(const int64_t* __restrict__ probs, double* __restrict__ A,
-                                                 double* __restrict__ Lo, double* __restrict__ Qo) {
+                                                 double* __restrict__ Lo, double* __restrict__ Qo,
+                                                 int64_t lds_doubles) {
   extern __shared__ double lds[];
   __shared__ double part[4];
   __shared__ double coef_s[2];
@@ -479,6 +480,9 @@ __global__ __launch_bounds__(256) void lq_kernel(const int64_t* __restrict__ pro
   double* L = Lo + pw[3];
   double* Q = Qo + pw[4];
   const int k = m < n ? m : n;
+  // a problem whose reflector and tau/beta arrays exceed the launch's LDS is left untouched
+  // (the caller sizes the launch for its largest problem)
+  if (static_cast<int64_t>(n) + 2 * static_cast<int64_t>(k) > lds_doubles) return;
   double* v = lds;          // n: the reflector being applied
   double* tau = lds + n;    // k
   double* beta = tau + k;   // k
@@ -646,6 +650,7 @@ extern "C" int tn_lq(const int64_t* probs, int64_t nprobs, int64_t max_lds_doubl
   const size_t bytes = static_cast<size_t>(max_lds_doubles) * sizeof(double);
   if (int rc = allow_lds(reinterpret_cast<const void*>(lq_kernel), bytes, "tn_lq")) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(lq_kernel, dim3(static_cast<unsigned>(nprobs)), dim3(256), bytes, s, probs, A, L, Q);
+  hipLaunchKernelGGL(lq_kernel, dim3(static_cast<unsigned>(nprobs)), dim3(256), bytes, s, probs, A, L, Q,
+                     max_lds_doubles);
   return tn::check_launch("tn_lq");
 }
