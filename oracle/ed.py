@@ -4,8 +4,13 @@ chains, the independent pins for the DMRG path.
 * ``xxz_sz0_energy`` -- sparse exact diagonalization in the S^z_tot = 0 block, the
   generalization to XXZ of the reference's ``heisenberg_sz0`` oracle
   (``tests/integration/test_dmrg.py:120-139``), ``H = sum_i (delta Sz Sz + (S+S- + S-S+)/2)``.
+* ``xxz_sz0_levels`` -- the ``k`` lowest levels of the same block (dense ``eigvalsh``), the
+  pins of the excited-state sweep (reference ``dmrg.py:620-843``, ``orthogonal_to``).
 * ``xx_energy`` -- closed form at delta = 0 (free fermions by Jordan-Wigner):
   ``E0 = sum_{k: eps_k < 0} eps_k``, ``eps_k = cos(pi k / (L+1))``.
+* ``xx_first_excited`` -- the lowest excitation at fixed particle number in the same closed
+  form: the highest occupied mode ``k = L/2 + 1`` moved to the lowest empty one ``k = L/2``,
+  ``E1 = E0 + eps_{L/2} - eps_{L/2+1} = E0 + 2 cos(pi L / (2 (L+1)))``.
 * Reference literals: ``E_OBC_12``, ``E_OBC_20`` (``test_dmrg.py:82-83``) and the MPSKit
   ``N32``/``N64`` energies of ``tests/fixtures/mpskit_heisenberg.json``.
 """
@@ -24,7 +29,7 @@ E_MPSKIT_N32 = -13.997315618224897
 E_MPSKIT_N64 = -28.175424859745323
 
 
-def xxz_sz0_energy(n_sites: int, delta: float = 1.0) -> float:
+def _xxz_sz0_matrix(n_sites: int, delta: float):
     if n_sites % 2:
         raise ValueError("S^z_tot = 0 needs an even chain")
     states = []
@@ -50,7 +55,16 @@ def xxz_sz0_energy(n_sites: int, delta: float = 1.0) -> float:
         cols.append(i)
         vals.append(diag)
     n = len(states)
-    h = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def xxz_sz0_levels(n_sites: int, delta: float = 1.0, k: int = 2) -> np.ndarray:
+    return np.linalg.eigvalsh(_xxz_sz0_matrix(n_sites, delta).toarray())[:k]
+
+
+def xxz_sz0_energy(n_sites: int, delta: float = 1.0) -> float:
+    h = _xxz_sz0_matrix(n_sites, delta)
+    n = h.shape[0]
     if n < 200:
         return float(np.linalg.eigvalsh(h.toarray())[0])
     return float(spla.eigsh(h, k=1, which="SA", tol=1e-14)[0][0])
@@ -59,3 +73,9 @@ def xxz_sz0_energy(n_sites: int, delta: float = 1.0) -> float:
 def xx_energy(n_sites: int) -> float:
     eps = np.cos(np.pi * np.arange(1, n_sites + 1) / (n_sites + 1))
     return float(eps[eps < 0].sum())
+
+
+def xx_first_excited(n_sites: int) -> float:
+    if n_sites % 2:
+        raise ValueError("S^z_tot = 0 needs an even chain")
+    return xx_energy(n_sites) + 2.0 * float(np.cos(np.pi * n_sites / (2 * (n_sites + 1))))

@@ -72,13 +72,18 @@ def lanczos(matvec: Callable[[SymmetricTensor], SymmetricTensor], v: SymmetricTe
                 "the start vector lies in the span of orthogonal_to, so there is no vector "
                 "left to iterate on at this bond"
             )
-        full = matvec
+        def projected(op):
+            def apply(x):
+                y = op(x)
+                for b in basis:
+                    y = ops.subtract(y, ops.multiply(b, ops.inner(b, y)))
+                return y
+            return apply
 
-        def matvec(x):
-            y = full(x)
-            for b in basis:
-                y = ops.subtract(y, ops.multiply(b, ops.inner(b, y)))
-            return y
+        # the FP64 refinement step runs on the same projected operator
+        matvec = projected(matvec)
+        if refine is not None:
+            refine = projected(refine)
 
     dev = v.device
     alphas = torch.zeros(ncv, dtype=torch.float64, device=dev)
@@ -184,12 +189,14 @@ def _pair(left, right):
 
 class LanczosRequest(NamedTuple):
     """One bond's Krylov step, handed to whoever solves it: the compiled operator, the FP64
-    refinement operator (or None), the start vector and the Krylov dimension."""
+    refinement operator (or None), the start vector, the Krylov dimension and the vectors the
+    solve stays orthogonal to (an excited-state sweep's projection vectors)."""
 
     op: Any
     refine: Any
     aa: SymmetricTensor
     ncv: int
+    ortho: tuple = ()
 
 
 def _rho(m: SymmetricTensor, forward: bool) -> SymmetricTensor:
@@ -237,7 +244,8 @@ def _bond_gen(psi, env, schmidt, n, forward, *, chi, cutoff, ncv, noise, seed, p
     aa = _pair(psi[n], psi[n + 1])
     op = env.operator(n, precision)
     ref = env.operator(n, "fp64") if (precision != "fp64" and refine) else None
-    energy, aa = yield LanczosRequest(op, ref, aa, ncv)
+    ortho = tuple(o.project2(n, _pair(o.psi[n], o.psi[n + 1])) for o in orthogonal_to)
+    energy, aa = yield LanczosRequest(op, ref, aa, ncv, ortho)
     if noise and noise_type == "perturbative":
         left, right, s = _split_dm(aa, forward, chi=chi, cutoff=cutoff,
                                    perturbations=_perturbations(env, n, aa, noise))
@@ -290,7 +298,8 @@ def _bond_step(psi, env, schmidt, n, forward, **kw):
     g = _bond_gen(psi, env, schmidt, n, forward, **kw)
     req = next(g)
     try:
-        g.send(lanczos(req.op, req.aa, ncv=req.ncv, refine=req.refine, host_value=False))
+        g.send(lanczos(req.op, req.aa, ncv=req.ncv, refine=req.refine, orthogonal_to=req.ortho,
+                       host_value=False))
     except StopIteration as stop:
         return stop.value
     raise RuntimeError("a bond step asked for more than one Lanczos solve")
@@ -305,6 +314,9 @@ def sweep_steps(psi, h, env, schmidt: dict, *, chi: int, cutoff: float, ncv: int
     scheduler can solve the Krylov steps of many solves together."""
     if noise_type not in ("wavefunction", "perturbative"):
         raise ValueError(f"noise_type={noise_type!r} is not one of ('wavefunction', 'perturbative')")
+    if external and orthogonal_to:
+        raise ValueError("an excited-state sweep (orthogonal_to) solves its own projected Krylov "
+                         "steps; it cannot hand them out (external=True)")
     n_sites = len(psi)
     kw = dict(chi=chi, cutoff=cutoff, ncv=ncv, noise=noise, seed=seed, precision=precision,
               refine=refine, orthogonal_to=orthogonal_to, noise_type=noise_type)
@@ -351,7 +363,8 @@ def _schmidt_change(old: dict, new: dict) -> float:
 def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1e-12,
                schmidt_tol: float = 1e-8, max_sweeps: int = 40, ncv: int = 3, seed: int = 0,
                callback=None, compile=None, precision: str = "fp64", refine: bool = True,
-               polish_tol: float | None = None, external: bool = False, gc_freeze: bool = True):
+               polish_tol: float | None = None, external: bool = False, gc_freeze: bool = True,
+               orthogonal_to=None):
     """``dmrg_`` as a generator: yields ``None`` after every bond and returns the
     ``DMRG_out`` (``StopIteration.value``) when the loop ends.  With ``external`` every
     yield is instead one bond's ``LanczosRequest`` and the caller sends back
@@ -381,6 +394,23 @@ def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: floa
         plan = [Sweep(64 if chi is None else chi, 1e-14 if cutoff is None else cutoff)]
     psi.canonize_(0)
     env = Env(psi, h, compile=compile).setup_(0)
+    # one two-state environment <psi|1|phi> per converged state, set up after psi's
+    # canonize_ since psi is its bra and it follows psi through the sweep (reference
+    # dmrg.py:787-806); a state whose boundary legs put it in another sector is orthogonal
+    # to psi by the symmetry and is skipped
+    orthos = ()
+    if orthogonal_to:
+        from tenet.network.mps import MPO
+
+        ident = MPO.identity(len(psi), psi[0].legs[1].space, device=psi[0].device)
+        ends = (psi[0].legs[0], psi[len(psi) - 1].legs[2])
+
+        def same_ends(phi):
+            a, b = phi[0].legs[0], phi[len(phi) - 1].legs[2]
+            return ((a.space, a.dual) == (ends[0].space, ends[0].dual)
+                    and (b.space, b.dual) == (ends[1].space, ends[1].dual))
+
+        orthos = tuple(Env(phi, ident, bra=psi).setup_(0) for phi in orthogonal_to if same_ends(phi))
     schmidt: dict = {}
     energy = float("inf")
     history, realized = [], []
@@ -390,7 +420,7 @@ def dmrg_steps(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: floa
         old_e, old_s = energy, dict(schmidt)
         max_dw = 0.0
         steps = sweep_steps(psi, h, env, schmidt, chi=entry.chi, cutoff=entry.cutoff, ncv=ncv,
-                            noise=entry.noise, noise_type=entry.noise_type,
+                            noise=entry.noise, noise_type=entry.noise_type, orthogonal_to=orthos,
                             seed=seed + 977 * it, precision=precision, refine=refine,
                             external=external)
         answer = None
@@ -428,13 +458,16 @@ def dmrg_(psi, h, *, schedule=None, chi=None, cutoff=None, energy_tol: float = 1
           schmidt_tol: float = 1e-8, max_sweeps: int = 40, ncv: int = 3, orthogonal_to=None,
           seed: int = 0, callback=None, compile=None, precision: str = "fp64",
           refine: bool = True, polish_tol: float | None = None) -> DMRG_out:
-    """Sweep ``psi`` to the ground state of ``h`` in place (reference ``dmrg.py:620-843``)."""
-    if orthogonal_to:
-        raise NotImplementedError("excited-state projection (orthogonal_to) is not implemented")
+    """Sweep ``psi`` to the ground state of ``h`` in place (reference ``dmrg.py:620-843``).
+
+    ``orthogonal_to``: converged states ``phi`` (MPS, held fixed) the result is kept
+    orthogonal to -- every bond's Krylov solve runs on ``H`` with the projection vectors
+    ``<psi env|1|phi>`` of the two-site space projected out, so ``psi`` converges to the
+    lowest state of ``h`` orthogonal to them (an excited state)."""
     gen = dmrg_steps(psi, h, schedule=schedule, chi=chi, cutoff=cutoff, energy_tol=energy_tol,
                      schmidt_tol=schmidt_tol, max_sweeps=max_sweeps, ncv=ncv, seed=seed,
                      callback=callback, compile=compile, precision=precision, refine=refine,
-                     polish_tol=polish_tol)
+                     polish_tol=polish_tol, orthogonal_to=orthogonal_to)
     while True:
         try:
             next(gen)

@@ -110,6 +110,29 @@ class Env:
         """``H_eff @ aa`` on bond ``(n, n+1)``; the result has ``aa``'s structure."""
         return heff2_full(self.h, self.F[n - 1, n], self.F[n + 2, n + 1], n, aa)
 
+    def project2(self, n: int, aa: SymmetricTensor) -> SymmetricTensor:
+        """``<bra env| W_n W_{n+1} |aa, ket env>`` on a two-state ``Env`` (reference
+        ``env.py:843-906``): ``aa`` is the ket chain's two-site tensor at bond ``(n, n+1)``
+        and the result lives on the bra chain's two-site space.  Over ``MPO.identity`` it
+        is the ket state read in the bra's environment gauge, the vector an excited-state
+        sweep projects out of the bra's variational space."""
+        return heff2_full(self.h, self.F[n - 1, n], self.F[n + 2, n + 1], n, aa)
+
+    def measure(self) -> float:
+        """``<bra|H|psi>``, not divided by either norm, by a left-to-right pass in a fresh
+        ``Env`` (the sweep's cache is not touched; reference ``env.py:948-977``).  The pass
+        ends on the right boundary, whose three legs carry one sector each: the closing
+        contraction with that boundary is the qdim-weighted sum of its entries."""
+        n = len(self.psi)
+        env = Env(self.psi, self.h, bra=self.bra)
+        for site in range(n):
+            env.update_(site, to="last")
+        last = env.F[n - 1, n]
+        for leg in last.legs:
+            if len(leg.space.sectors) != 1 or leg.space.dim != leg.space.provider.qdim(leg.space.sectors[0][0]):
+                raise ValueError("measure() closes on boundary legs of one degeneracy-1 sector each")
+        return float(ops.inner(ones(last.legs, dtype=last.dtype, device=last.device), last))
+
     def heff2_families(self, n: int, aa: SymmetricTensor) -> tuple:
         """``heff2``'s term families applied separately (reference ``env.py:772-841``).  The
         two-site H_eff here always runs on the MPO's site tensors, where the operator has

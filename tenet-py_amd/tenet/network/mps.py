@@ -172,6 +172,16 @@ class MPO:
         return cls(sites, edges=table)
 
     @classmethod
+    def identity(cls, n_sites: int, phys: GradedSpace, *, device=None) -> "MPO":
+        """The identity operator: unit-sector ``D = 1`` bonds and ``eye`` on every site
+        (reference ``mps.py:2731-2778``).  A two-state ``Env`` over it gives overlaps and
+        the per-bond projection vectors of an excited-state sweep."""
+        unit = GradedSpace.new(phys.provider, {phys.provider.unit: 1})
+        legs = (Leg(unit, IN), Leg(phys, OUT), Leg(phys, IN), Leg(unit, OUT))
+        eye = from_dense(np.eye(phys.dim).reshape(1, phys.dim, phys.dim, 1), legs, device=device)
+        return cls([eye] * n_sites)
+
+    @classmethod
     def from_w(cls, w, n_sites: int, *, phys: GradedSpace, bond: GradedSpace,
                boundary: GradedSpace, start: int, end: int, device=None) -> "MPO":
         """One bulk ``W[wl, p_out, p_in, wr]`` with boundary rows/columns cut at ``start``

@@ -57,6 +57,20 @@ def test_ed_against_reference_literals_and_closed_form():
         assert abs(ed.xxz_sz0_energy(L, 0.0) - ed.xx_energy(L)) < 1e-10
 
 
+def test_ed_levels_against_closed_forms():
+    """The two lowest S^z = 0 levels: the free-fermion ground state and its lowest
+    particle-number-conserving excitation at delta = 0, and the two-site singlet/triplet
+    (-3/4, +1/4) and four-site ground state -3/4 - sqrt(3)/2 at delta = 1."""
+    for L in (6, 8, 10):
+        lv = ed.xxz_sz0_levels(L, 0.0, 2)
+        assert abs(lv[0] - ed.xx_energy(L)) < 1e-10
+        assert abs(lv[1] - ed.xx_first_excited(L)) < 1e-10
+        assert lv[1] - lv[0] > 1e-3  # a non-degenerate first level
+    assert np.allclose(ed.xxz_sz0_levels(2, 1.0, 2), [-0.75, 0.25], atol=1e-14)
+    assert abs(ed.xxz_sz0_levels(4, 1.0, 1)[0] - (-0.75 - 3 ** 0.5 / 2)) < 1e-13
+    assert abs(ed.xxz_sz0_levels(12, 1.0, 1)[0] - ed.E_OBC_12) < 1e-10
+
+
 def _heis(L, delta=1.0):
     import sys
     from pathlib import Path
