@@ -22,7 +22,7 @@ import torch
 
 from tenet.engine import RecoupleProgram, require_device, upload, wait_upload
 from tenet.ops import repartition
-from tenet.plans import map_layout
+from tenet.plans import band_layout
 from tenet.space import IN, OUT, GradedSpace, Leg
 from tenet.structure import TensorStructure
 from tenet.symmetry import StructureChangingError
@@ -40,33 +40,19 @@ class _MatrixLayout:
     """Packed sector matrices of a structure and the two recouple programs to/from them."""
 
     def __init__(self, s: TensorStructure) -> None:
-        lay = map_layout(s)
-        self.sectors = lay.sectors
-        rows_of, cols_of = {c: [] for c in lay.sectors}, {c: [] for c in lay.sectors}
-        for c, _, off, ext in lay.rows:
-            rows_of[c].append((off, ext))
-        for c, _, off, ext in lay.cols:
-            cols_of[c].append((off, ext))
-        self.shapes = {c: (sum(e for _, e in rows_of[c]), sum(e for _, e in cols_of[c])) for c in lay.sectors}
-        self.offsets = {}
-        acc = 0
-        for c in lay.sectors:
-            self.offsets[c] = acc
-            r, k = self.shapes[c]
-            acc += r * k
-        self.size = acc
+        pat, _, roff, _, coff, rtot, ctot = band_layout(s)
+        self.sectors = pat.sectors
+        self.shapes = {c: (int(r), int(k)) for c, r, k in zip(pat.sectors, rtot.tolist(), ctot.tolist())}
+        sizes = rtot * ctot
+        base = np.zeros(len(sizes), dtype=np.int64)
+        if len(sizes) > 1:
+            np.cumsum(sizes[:-1], out=base[1:])
+        self.offsets = dict(zip(pat.sectors, base.tolist()))
+        self.size = int(sizes.sum())
         # one region per block: its matrix offset (row band x column band of its sector)
-        blk, moff = [], []
-        for c, cells in lay.grid:
-            ld, base = self.shapes[c][1], self.offsets[c]
-            for (ro, _), row in zip(rows_of[c], cells):
-                for (co, _), i in zip(cols_of[c], row):
-                    blk.append(i)
-                    moff.append(base + ro * ld + co)
-        blk = np.asarray(blk, dtype=np.int64)
-        moff = np.asarray(moff, dtype=np.int64)
-        ld = np.asarray([self.shapes[c][1] for c, cells in lay.grid for row in cells for _ in row],
-                        dtype=np.int64)
+        blk = pat.blk
+        ld = ctot[pat.blk_sector]
+        moff = base[pat.blk_sector] + roff[pat.blk_row] * ld + coff[pat.blk_col]
         bl = s.layout
         order = [*s.out_axes, *s.in_axes]
         n_out = len(s.out_axes)

@@ -362,34 +362,104 @@ class MapLayout:
         )
 
 
-def _tree_extent(structure: TensorStructure, axes: tuple, tree) -> int:
-    """Rows (or columns) a tree's block occupies: the product of the degeneracies of its
-    uncoupled sectors on the legs of that side."""
-    return int(np.prod([structure.legs[a].degeneracy(structure.legs[a].space_sector(u))
-                        for a, u in zip(axes, tree.uncoupled)], dtype=np.int64))
+@dataclass(frozen=True)
+class _BandPattern:
+    """The degeneracy-free part of a map layout, shared by every structure of one block
+    pattern: the sectors; per row (column) band its sector's position, its tree and one
+    block lying in it; per block (in sector, row band, column band order) its block index,
+    sector position and band numbers; and the grid of block indices."""
+
+    sectors: tuple
+    row_sector: np.ndarray
+    row_tree: tuple
+    row_rep: np.ndarray
+    col_sector: np.ndarray
+    col_tree: tuple
+    col_rep: np.ndarray
+    blk: np.ndarray
+    blk_sector: np.ndarray
+    blk_row: np.ndarray
+    blk_col: np.ndarray
+    grid: tuple
+
+
+@cache
+def _band_pattern(pattern: TensorStructure) -> _BandPattern:
+    """Blocks of one coupled sector tile that sector's matrix: OUT trees index row bands,
+    IN trees column bands, both in order of first appearance in the block order (which is
+    sorted, so the bands are too).  Every (row tree, column tree) pair of a sector must be
+    a block, or the structure is not a per-sector cross product."""
+    per_sector: dict = {}  # c -> ({out tree: band}, {in tree: band}, {(band, band): block})
+    for i, key in enumerate(pattern.block_order):
+        r, k, cells = per_sector.setdefault(key.coupled, ({}, {}, {}))
+        cells[(r.setdefault(key.output_tree, len(r)), k.setdefault(key.input_tree, len(k)))] = i
+    sectors = tuple(sorted(per_sector))
+    row_sector, row_tree, row_rep, col_sector, col_tree, col_rep = [], [], [], [], [], []
+    blk, blk_sector, blk_row, blk_col, grid = [], [], [], [], []
+    for ci, c in enumerate(sectors):
+        r, k, cells = per_sector[c]
+        if len(cells) != len(r) * len(k):
+            raise RuntimeError("map_layout: block order is not a per-sector cross product")
+        r0, k0 = len(row_tree), len(col_tree)
+        rows = tuple(tuple(cells[(a, b)] for b in range(len(k))) for a in range(len(r)))
+        row_tree.extend(r)
+        row_sector.extend([ci] * len(r))
+        row_rep.extend(row[0] for row in rows)
+        col_tree.extend(k)
+        col_sector.extend([ci] * len(k))
+        col_rep.extend(rows[0])
+        for a, row in enumerate(rows):
+            blk.extend(row)
+            blk_sector.extend([ci] * len(row))
+            blk_row.extend([r0 + a] * len(row))
+            blk_col.extend(range(k0, k0 + len(row)))
+        grid.append((c, rows))
+    ix = lambda v: np.asarray(v, dtype=np.int64)  # noqa: E731
+    return _BandPattern(sectors, ix(row_sector), tuple(row_tree), ix(row_rep), ix(col_sector),
+                        tuple(col_tree), ix(col_rep), ix(blk), ix(blk_sector), ix(blk_row),
+                        ix(blk_col), tuple(grid))
+
+
+def _band_extents(shapes: np.ndarray, rep: np.ndarray, axes: tuple, sector: np.ndarray, nsec: int):
+    """Extents and in-sector offsets of bands: a band's extent is the product of the
+    degeneracies on that side's legs of any block in it; offsets run within each sector."""
+    if not len(rep):
+        z = np.zeros(0, dtype=np.int64)
+        return z, z, np.zeros(nsec, dtype=np.int64)
+    ext = np.prod(shapes[rep][:, list(axes)], axis=1, dtype=np.int64) if axes else np.ones(len(rep), np.int64)
+    total = np.zeros(nsec, dtype=np.int64)
+    np.add.at(total, sector, ext)
+    start = np.zeros(nsec, dtype=np.int64)
+    np.cumsum(total[:-1], out=start[1:])
+    off = np.cumsum(ext) - ext - start[sector]
+    return ext, off, total
+
+
+@cache
+def band_layout(structure: TensorStructure) -> tuple:
+    """``(pattern, row extents, row offsets, column extents, column offsets, sector rows,
+    sector columns)`` of a structure's map layout as int64 arrays (bands in the pattern's
+    order, offsets within their sector)."""
+    pat = _band_pattern(pattern_of(structure))
+    shapes = structure.layout.shapes
+    nsec = len(pat.sectors)
+    rext, roff, rtot = _band_extents(shapes, pat.row_rep, structure.out_axes, pat.row_sector, nsec)
+    cext, coff, ctot = _band_extents(shapes, pat.col_rep, structure.in_axes, pat.col_sector, nsec)
+    return pat, rext, roff, cext, coff, rtot, ctot
 
 
 @cache
 def map_layout(structure: TensorStructure) -> MapLayout:
-    """Blocks of one coupled sector tile that sector's matrix: OUT trees index row bands,
-    IN trees column bands, both in order of first appearance in the block order (which
-    is sorted, so the bands are too).  Every (row tree, column tree) pair of a sector must
-    be a block, or the structure is not a per-sector cross product."""
-    per_sector: dict = {}  # c -> ({out tree: band}, {in tree: band}, {(band, band): block})
-    for i, key in enumerate(structure.block_order):
-        r, k, cells = per_sector.setdefault(key.coupled, ({}, {}, {}))
-        cells[(r.setdefault(key.output_tree, len(r)), k.setdefault(key.input_tree, len(k)))] = i
-    rows, cols, grid = [], [], []
-    for c in sorted(per_sector):
-        r, k, cells = per_sector[c]
-        if len(cells) != len(r) * len(k):
-            raise RuntimeError("map_layout: block order is not a per-sector cross product")
-        for bands, axes, dst in ((r, structure.out_axes, rows), (k, structure.in_axes, cols)):
-            extents = [_tree_extent(structure, axes, t) for t in bands]
-            starts = np.concatenate([[0], np.cumsum(extents)[:-1]]).astype(np.int64) if extents else []
-            dst.extend((c, t, int(o), int(e)) for t, o, e in zip(bands, starts, extents))
-        grid.append((c, tuple(tuple(cells[(a, b)] for b in range(len(k))) for a in range(len(r)))))
-    return MapLayout(structure, tuple(sorted(per_sector)), tuple(rows), tuple(cols), tuple(grid))
+    """Row bands over OUT trees, column bands over IN trees and the block grid of every
+    coupled sector (the band pattern is shared by all structures of one block pattern;
+    only the extents are read from this structure's degeneracies)."""
+    pat, rext, roff, cext, coff, _, _ = band_layout(structure)
+    secs = pat.sectors
+    rows = tuple((secs[c], t, o, e) for c, t, o, e in zip(pat.row_sector.tolist(), pat.row_tree,
+                                                          roff.tolist(), rext.tolist()))
+    cols = tuple((secs[c], t, o, e) for c, t, o, e in zip(pat.col_sector.tolist(), pat.col_tree,
+                                                          coff.tolist(), cext.tolist()))
+    return MapLayout(structure, secs, rows, cols, pat.grid)
 
 
 # ----------------------------------------------------------------------------- contraction
