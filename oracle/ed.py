@@ -6,6 +6,8 @@ chains, the independent pins for the DMRG path.
   (``tests/integration/test_dmrg.py:120-139``), ``H = sum_i (delta Sz Sz + (S+S- + S-S+)/2)``.
 * ``xxz_sz0_levels`` -- the ``k`` lowest levels of the same block (dense ``eigvalsh``), the
   pins of the excited-state sweep (reference ``dmrg.py:620-843``, ``orthogonal_to``).
+* ``heisenberg_singlet_levels`` -- the total-spin-0 levels (the S^z = 0 spectrum less the
+  S^z = 1 spectrum), the pins of an SU(2) excited-state sweep.
 * ``xx_energy`` -- closed form at delta = 0 (free fermions by Jordan-Wigner):
   ``E0 = sum_{k: eps_k < 0} eps_k``, ``eps_k = cos(pi k / (L+1))``.
 * ``xx_first_excited`` -- the lowest excitation at fixed particle number in the same closed
@@ -32,8 +34,12 @@ E_MPSKIT_N64 = -28.175424859745323
 def _xxz_sz0_matrix(n_sites: int, delta: float):
     if n_sites % 2:
         raise ValueError("S^z_tot = 0 needs an even chain")
+    return _xxz_block(n_sites, delta, n_sites // 2)
+
+
+def _xxz_block(n_sites: int, delta: float, n_up: int):
     states = []
-    for ups in combinations(range(n_sites), n_sites // 2):
+    for ups in combinations(range(n_sites), n_up):
         s = 0
         for u in ups:
             s |= 1 << u
@@ -60,6 +66,25 @@ def _xxz_sz0_matrix(n_sites: int, delta: float):
 
 def xxz_sz0_levels(n_sites: int, delta: float = 1.0, k: int = 2) -> np.ndarray:
     return np.linalg.eigvalsh(_xxz_sz0_matrix(n_sites, delta).toarray())[:k]
+
+
+def heisenberg_singlet_levels(n_sites: int, k: int = 2) -> np.ndarray:
+    """The ``k`` lowest total-spin-0 levels of the open Heisenberg chain: every multiplet
+    of spin S >= 1 has one state in the S^z = 0 block and one in the S^z = 1 block, a
+    singlet only the first, so the singlet levels are the S^z = 0 spectrum with the
+    S^z = 1 spectrum removed level by level."""
+    e0 = np.linalg.eigvalsh(_xxz_block(n_sites, 1.0, n_sites // 2).toarray())
+    e1 = list(np.linalg.eigvalsh(_xxz_block(n_sites, 1.0, n_sites // 2 + 1).toarray()))
+    out = []
+    for e in e0:
+        j = next((i for i, x in enumerate(e1) if abs(x - e) < 1e-9), None)
+        if j is None:
+            out.append(float(e))
+            if len(out) == k:
+                break
+        else:
+            e1.pop(j)
+    return np.asarray(out)
 
 
 def xxz_sz0_energy(n_sites: int, delta: float = 1.0) -> float:

@@ -76,3 +76,21 @@ def test_external_solves_refuse_an_excited_state_sweep():
     psi = random_chain(6, 8, 3.0, seed=2, device="cuda")
     with pytest.raises(ValueError):
         next(dmrg_steps(psi, h, chi=8, external=True, orthogonal_to=[phi]))
+
+
+def test_su2_first_excited_singlet():
+    """SU(2) chains hold total spin 0 only: the excited-state sweep finds the second singlet
+    level of the L=10 Heisenberg chain (ED: the S^z = 0 spectrum less the S^z = 1 one)."""
+    from tenet.network import MPO, Env, dmrg_
+    from tenet.network.models import SU2_PHYS, su2_heisenberg_mpo, su2_random_chain
+
+    L = 10
+    h = su2_heisenberg_mpo(L, device="cuda")
+    psi0 = su2_random_chain(L, 8, 3.0, seed=1, device="cuda")
+    out0 = dmrg_(psi0, h, chi=48, max_sweeps=12)
+    psi1 = su2_random_chain(L, 8, 3.0, seed=2, device="cuda")
+    out1 = dmrg_(psi1, h, chi=48, max_sweeps=16, orthogonal_to=[psi0])
+    e0, e1 = ed.heisenberg_singlet_levels(L, 2)
+    assert abs(out0.energy - e0) < 1e-10 * abs(e0)
+    assert abs(out1.energy - e1) < 1e-10 * abs(e1), (out1.energy, e1)
+    assert abs(Env(psi1, MPO.identity(L, SU2_PHYS, device="cuda"), bra=psi0).measure()) < 1e-6

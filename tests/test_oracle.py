@@ -71,6 +71,20 @@ def test_ed_levels_against_closed_forms():
     assert abs(ed.xxz_sz0_levels(12, 1.0, 1)[0] - ed.E_OBC_12) < 1e-10
 
 
+def test_singlet_levels_against_closed_forms():
+    """Two sites: the singlet -3/4 alone.  Four sites (S1.S2 + S2.S3 + S3.S4): the two
+    singlets -3/4 -+ sqrt(3)/2 (the S^z = 0 block also holds the triplets -1/4 -+ 1/sqrt(2)
+    and -1/4 and the quintet 3/4, all of which reappear at S^z = 1).  Twelve sites: the
+    ground state is the singlet E_OBC_12."""
+    sz0 = np.linalg.eigvalsh(ed._xxz_block(4, 1.0, 2).toarray())
+    assert np.allclose(sz0, sorted([-0.75 - 3 ** 0.5 / 2, -0.75 + 3 ** 0.5 / 2, -0.25 - 0.5 ** 0.5,
+                                    -0.25, -0.25 + 0.5 ** 0.5, 0.75]), atol=1e-13)
+    assert np.allclose(ed.heisenberg_singlet_levels(2, 2), [-0.75], atol=1e-14)
+    assert np.allclose(ed.heisenberg_singlet_levels(4, 2), [-0.75 - 3 ** 0.5 / 2, -0.75 + 3 ** 0.5 / 2],
+                       atol=1e-13)
+    assert abs(ed.heisenberg_singlet_levels(12, 1)[0] - ed.E_OBC_12) < 1e-10
+
+
 def _heis(L, delta=1.0):
     import sys
     from pathlib import Path
