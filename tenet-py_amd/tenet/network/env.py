@@ -22,7 +22,7 @@ from tenet.space import IN, OUT, Leg
 from tenet.structure import TensorStructure
 from tenet.tensor import SymmetricTensor
 
-__all__ = ["Env", "ones", "composed", "heff2_full"]
+__all__ = ["Env", "ones", "composed", "heff2_full", "measure_mpo"]
 
 
 def ones(legs, *, dtype=torch.float64, device=None) -> SymmetricTensor:
@@ -152,3 +152,9 @@ class Env:
             op = Heff2(fl, self.h[n], self.h[n + 1], fr, precision)
             self._ops[key] = op
         return op
+
+
+def measure_mpo(bra, h, ket) -> float:
+    """``<bra|H|ket>``, undivided by either norm (reference ``env.py:996-1040``): the closing
+    contraction of a fresh two-state ``Env`` pass."""
+    return Env(ket, h, bra=bra).measure()

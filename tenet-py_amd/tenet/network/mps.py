@@ -19,7 +19,7 @@ from tenet.space import IN, OUT, GradedSpace, Leg
 from tenet.structure import TensorStructure
 from tenet.tensor import SymmetricTensor, default_device
 
-__all__ = ["MPS", "MPO", "from_dense"]
+__all__ = ["MPS", "MPO", "from_dense", "overlap"]
 
 
 def _as_site(t: SymmetricTensor) -> SymmetricTensor:
@@ -124,6 +124,17 @@ class MPS:
         self[0] = ops.divide(self[0], ops.norm(self[0]))
         self.center = to
         return self
+
+
+def overlap(bra: MPS, ket: MPS) -> float:
+    """``<bra|ket>``, undivided by either norm (reference ``mps.py:714-760``): the two-state
+    environment pass over ``MPO.identity``."""
+    from tenet.network.env import Env
+
+    if len(bra) != len(ket):
+        raise ValueError(f"overlap: chains of {len(bra)} and {len(ket)} sites")
+    ident = MPO.identity(len(ket), ket[0].legs[1].space, device=ket[0].device)
+    return Env(ket, ident, bra=bra).measure()
 
 
 def from_dense(array, legs: Sequence[Leg], *, atol: float = 1e-10, device=None) -> SymmetricTensor:

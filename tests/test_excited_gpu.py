@@ -94,3 +94,19 @@ def test_su2_first_excited_singlet():
     assert abs(out0.energy - e0) < 1e-10 * abs(e0)
     assert abs(out1.energy - e1) < 1e-10 * abs(e1), (out1.energy, e1)
     assert abs(Env(psi1, MPO.identity(L, SU2_PHYS, device="cuda"), bra=psi0).measure()) < 1e-6
+
+
+def test_overlap_and_measure_mpo():
+    """``overlap`` against the oracle's dense contraction; ``measure_mpo(psi, h, psi)`` of a
+    converged ground state is its energy times ``<psi|psi>``."""
+    from tenet.network import dmrg_, measure_mpo, overlap
+    from tenet.network.models import heisenberg_mpo, random_chain
+
+    psi = random_chain(8, 12, 2.0, seed=21, device="cuda")
+    phi = random_chain(8, 12, 2.0, seed=22, device="cuda")
+    ref = _dense_overlap(phi, psi)
+    assert abs(overlap(phi, psi) - ref) <= 1e-12 * max(abs(ref), 1.0)
+    h = heisenberg_mpo(10, device="cuda")
+    g = random_chain(10, 16, 3.0, seed=1, device="cuda")
+    out = dmrg_(g, h, chi=32, max_sweeps=12)
+    assert abs(measure_mpo(g, h, g) - out.energy * overlap(g, g)) <= 1e-10 * abs(out.energy)
