@@ -237,6 +237,10 @@ _SVD_INNER_SWEEPS = 1     # inner sweeps per block-pair visit when a problem spa
 _SVD_SINGLE_SWEEPS = 64   # cap for a problem that fits in one block pair (converges in-launch)
 _SVD_LAST_SWEEPS = 0      # sweeps queued by the last call (diagnostics)
 _SVD_SWEEP_BATCH = 8      # sweeps queued before the first read of the convergence flags
+# per launch-group shape ((w, rounds, single) of every group): the sweeps the last such SVD
+# had queued before its final batch, plus one -- the first batch of the next one, so that a
+# run of similar bonds (a DMRG sweep) reads the flags about once per SVD
+_SVD_SWEEP_HINT: dict = {}
 # widest block of a block tournament (vectors): a round's time follows the pairs one
 # workgroup rotates, and the sweep's depth (about k inner rounds) does not depend on the
 # width, so narrower blocks spread the same rounds over more workgroups (16 measured
@@ -413,9 +417,11 @@ def _svd_run(packed, complete: bool):
         side.wait_stream(main)
     streams = [side.cuda_stream if (side is not None and g["single"]) else stream for g in plan.groups]
     global _SVD_LAST_SWEEPS
+    shape_key = tuple((g["w"], g["rounds"], g["single"]) for g in plan.groups)
+    first = max(_SVD_SWEEP_BATCH, _SVD_SWEEP_HINT.get(shape_key, 0))
     done = 0
     while True:
-        batch = min(_SVD_SWEEP_BATCH if done == 0 else _SVD_SWEEP_BATCH // 2, _SVD_MAX_SWEEPS - done)
+        batch = min(first if done == 0 else _SVD_SWEEP_BATCH // 2, _SVD_MAX_SWEEPS - done)
         for sweep in range(done, done + batch):
             for gi, (g, rot, st) in enumerate(zip(plan.groups, rots, streams)):
                 flag = stop[gi:]
@@ -435,6 +441,8 @@ def _svd_run(packed, complete: bool):
         if done >= _SVD_MAX_SWEEPS:
             raise RuntimeError(f"svd: the Jacobi sweeps did not converge in {_SVD_MAX_SWEEPS} sweeps")
     _SVD_LAST_SWEEPS = done
+    # converged inside the last batch: at least done - batch + 1 sweeps were needed
+    _SVD_SWEEP_HINT[shape_key] = done - batch + 1
     U = torch.empty(plan.u_size, dtype=torch.float64, device=dev)
     S = torch.empty(plan.s_size, dtype=torch.float64, device=dev)
     Vh = torch.empty(plan.v_size, dtype=torch.float64, device=dev)
