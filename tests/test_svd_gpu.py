@@ -163,3 +163,35 @@ def test_truncating_form_keeps_the_values_of_the_exact_one():
         live = st > 1e-6 * st.max()  # well separated values: vectors fixed up to sign
         ue, ut = exact[c][0].cpu().numpy()[:, live], trunc[c][0].cpu().numpy()[:, live]
         assert np.abs(np.abs(np.sum(ue * ut, axis=0)) - 1.0).max() < 1e-9
+
+
+@pytest.mark.parametrize("complete", [True, False])
+def test_sweeps_queued_past_convergence_change_nothing(complete):
+    """The first batch of queued Jacobi sweeps follows the previous SVD of the same launch
+    shape; sweeps queued after the convergence flag is set must leave every factor
+    bit-identical (a block tournament and single-workgroup problems in one call)."""
+    from tenet import linalg
+
+    rng = np.random.default_rng(5)
+    shapes = [(130, 128), (40, 60), (96, 100)]
+    ml = _Layout(shapes)
+    flat = rng.standard_normal(ml.size)
+    buf = torch.from_numpy(flat).cuda()
+    saved = dict(linalg._SVD_SWEEP_HINT)
+    try:
+        linalg._SVD_SWEEP_HINT.clear()
+        base = linalg._svd_parts((buf, ml), complete=complete)
+        queued_small = linalg._SVD_LAST_SWEEPS
+        for key in list(linalg._SVD_SWEEP_HINT):
+            linalg._SVD_SWEEP_HINT[key] = 50
+        long = linalg._svd_parts((buf, ml), complete=complete)
+        assert linalg._SVD_LAST_SWEEPS >= 50 > queued_small
+    finally:
+        linalg._SVD_SWEEP_HINT.clear()
+        linalg._SVD_SWEEP_HINT.update(saved)
+    for c in base:
+        for x, y in zip(base[c], long[c]):
+            assert torch.equal(x, y), c
+    if complete:
+        _check({c: flat[ml.offsets[c]: ml.offsets[c] + m * n].reshape(m, n) for c, (m, n) in enumerate(shapes)},
+               long)
